@@ -1,0 +1,264 @@
+/*
+ * nbody_amd.h -- C ABI of the MI355X (gfx950) N-body force and integration
+ * engine.  Plain pointers and sizes only; every call returns a status code
+ * (NB_OK == 0) and never exits the process.  The message of the last failing
+ * call on the calling thread is returned by nb_last_error().
+ *
+ * Ownership: device buffers inside nb_particles are owned by the library
+ * (allocated by nb_particles_alloc); host arrays passed to upload/download are
+ * owned by the caller.  Force calculators, integrators, ensembles and servers
+ * own their device workspaces.
+ *
+ * Ordering: compute calls are asynchronous and ordered on the calculator's
+ * HIP stream; nb_particles_download() and nb_synchronize() wait for all work
+ * on the device.  One calculator/integrator must not be used from two host
+ * threads at once.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository).
+ */
+#ifndef NBODY_AMD_H
+#define NBODY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum nb_status {
+    NB_OK = 0,
+    NB_ERR_INVALID = 1,  /* bad argument or shape */
+    NB_ERR_HIP = 2,      /* HIP runtime / launch error */
+    NB_ERR_NOMEM = 3,    /* device allocation failed */
+    NB_ERR_CAPACITY = 4, /* tree depth / walk frontier capacity exceeded (reported, never silent) */
+    NB_ERR_BUDGET = 5,   /* HBM budget exceeded (server admission) */
+    NB_ERR_STATE = 6     /* call not valid in the object's current state */
+};
+
+/* Message of the last failing call on this thread ("" if none).
+ * Replaces the reference's exit()/assert() error paths
+ * (CUDA/src/force_calculator_naive.cu:86-90, CPU/src/force_calculator_barnes_hut_optimized.cpp:189). */
+const char* nb_last_error(void);
+
+/* Number of visible devices and selection of the current one. */
+int nb_device_count(int* count);
+int nb_set_device(int device);
+/* Waits for all outstanding work on the current device. */
+int nb_synchronize(void);
+/* Library version string. */
+const char* nb_version(void);
+
+/* ---------------------------------------------------------- particle state */
+/* Structure-of-arrays device state, FP64 throughout.  Replaces ParticleArrays
+ * (CUDA/include/particle_data.hpp:6-43); the reference's Morton/sorted scratch
+ * arrays live in the Barnes-Hut calculator's workspace instead. */
+typedef struct nb_particles {
+    int64_t n;
+    double *x, *y, *z;    /* positions */
+    double *vx, *vy, *vz; /* velocities */
+    double *ax, *ay, *az; /* accelerations written by nb_force_compute */
+    double *m;            /* masses */
+} nb_particles;
+
+/* allocateParticles (CUDA/include/particle_data.hpp:45-47, CUDA/src/particle_data.cu:6-43).
+ * Accelerations are zeroed. */
+int nb_particles_alloc(nb_particles* p, int64_t n);
+/* freeParticles (CUDA/include/particle_data.hpp:49-50). */
+int nb_particles_free(nb_particles* p);
+/* copyParticlesToDevice (CUDA/include/particle_data.hpp:52-61, CUDA/src/particle_data.cu:95-119).
+ * Velocity pointers may be NULL (zero velocities). */
+int nb_particles_upload(nb_particles* p, const double* x, const double* y, const double* z,
+                        const double* vx, const double* vy, const double* vz, const double* m);
+/* copyParticlesToHost (CUDA/include/particle_data.hpp:63-73, CUDA/src/particle_data.cu:122-143).
+ * Any pointer may be NULL to skip that array.  Synchronizes the device first. */
+int nb_particles_download(const nb_particles* p, double* x, double* y, double* z, double* vx,
+                          double* vy, double* vz, double* ax, double* ay, double* az, double* m);
+
+/* -------------------------------------------------------- force calculators */
+typedef struct nb_force nb_force;
+
+enum nb_direct_precision {
+    /* FP64, bit-identical to CPU/src/force_calculator_naive.cpp:20-30
+     * (IEEE sqrt and division, no FMA contraction, j ascending, j != i). */
+    NB_DIRECT_F64_EXACT = 0,
+    /* FP64 with rsqrt + one Newton step and FMA (fast FP64 reference). */
+    NB_DIRECT_F64 = 1,
+    /* FP32 LDS-tiled wave64 kernel with v_rsq_f32; FP64 results. */
+    NB_DIRECT_F32 = 2
+};
+
+/* ForceCalculatorNaive(double softening) (CPU/include/force_calculator_naive.hpp:12,
+ * CUDA/include/force_calculator_naive.hpp:13).  G is explicit here; the reference
+ * fixes it at 4*pi^2 (CPU/include/common.hpp:28).  softening is the length eps
+ * (squared internally, as CPU/include/force_calculator.hpp:19). */
+int nb_force_create_direct(nb_force** f, int precision, double G, double softening);
+
+enum nb_multipole { NB_MONOPOLE = 0, NB_QUADRUPOLE = 1 };
+
+/* BarnesHutForceCalculator(double softening, double theta, int leaf_capacity)
+ * (CUDA/include/force_calculator_barnes_hut.hpp:56) /
+ * ForceCalculatorBarnesHutOptimized(double, double, int)
+ * (CPU/include/force_calculator_barnes_hut_optimized.hpp:40).
+ * Tree topology follows the CPU reference (root cube centred on the bbox
+ * midpoint, octant = x>=c | y>=c<<1 | z>=c<<2, split when count > leaf_capacity). */
+int nb_force_create_bh(nb_force** f, double G, double softening, double theta, int leaf_capacity,
+                       int multipole);
+int nb_force_destroy(nb_force* f);
+
+/* ForceCalculator::computeAccelerations(ParticleArrays&)
+ * (CUDA/include/force_calculator.hpp:25-27; CPU form CPU/include/force_calculator.hpp:24-26).
+ * Writes p->ax, p->ay, p->az. */
+int nb_force_compute(nb_force* f, nb_particles* p);
+
+/* ForceCalculator::name() (CPU/include/force_calculator.hpp:28). */
+const char* nb_force_name(const nb_force* f);
+
+/* Timing of the calls so far, in ms measured with HIP events
+ * (getTimeStatistics, CPU/include/force_calculator.hpp:30; CUDA
+ * BarnesHutForceCalculator getters, CUDA/include/force_calculator_barnes_hut.hpp:120-125).
+ * out[0]=total, out[1]=tree build (bbox..up-sweep), out[2]=walk, out[3]=calls.
+ * Timing events are recorded only when enabled (they add stream syncs). */
+int nb_force_set_timing(nb_force* f, int enabled);
+int nb_force_timing(const nb_force* f, double out[4]);
+int nb_force_reset_timing(nb_force* f);
+
+/* Statistics of the last Barnes-Hut evaluation (synchronizes). */
+typedef struct nb_bh_stats {
+    int64_t n_nodes;        /* non-empty tree nodes */
+    int64_t n_leaves;       /* non-empty leaves */
+    int32_t max_level;      /* deepest occupied level */
+    int32_t level_limit;    /* deepest level the 63-bit keys / 1e-9 split rule allow */
+    int64_t oversize_cells; /* leaves at the key-depth limit holding > leaf_capacity
+                               (the CPU reference would keep splitting them) */
+    int64_t pp_interactions;   /* particle-particle interactions in the walk */
+    int64_t pc_interactions;   /* particle-cell interactions in the walk */
+    int32_t max_frontier;      /* largest walk frontier seen */
+    int32_t frontier_overflow; /* nonzero if the walk frontier overflowed (result invalid) */
+    double root_center[3];
+    double root_size;
+    int64_t n_groups;          /* walk groups (tree cells of <= 64 particles) */
+} nb_bh_stats;
+int nb_force_bh_stats(nb_force* f, nb_bh_stats* out);
+
+/* Topology of the last Barnes-Hut tree, per particle in ORIGINAL order: depth
+ * of the leaf holding it and its octant path (3 bits per level, root first).
+ * Also the 63-bit keys (original order) and the sort permutation (sorted
+ * position -> original index).  Any output pointer may be NULL.  Synchronizes. */
+int nb_force_bh_topology(nb_force* f, int64_t n, int32_t* leaf_depth, uint64_t* leaf_path,
+                         uint64_t* keys, uint32_t* perm);
+
+/* Multipole moments of every non-empty node of the last tree (synchronizes).
+ * Nodes are in level order, key order inside a level.  For each node:
+ * level, path, particle range [first,last) in sorted order, mass, centre of
+ * mass (3), traceless quadrupole about the COM (xx,yy,zz,xy,xz,yz).  Pass
+ * capacity = number of slots in the arrays; *count receives the node count. */
+int nb_force_bh_nodes(nb_force* f, int64_t capacity, int64_t* count, int32_t* level,
+                      uint64_t* path, int32_t* first, int32_t* last, double* mass, double* com,
+                      double* quad);
+
+/* -------------------------------------------------------------- integrators */
+typedef struct nb_integrator nb_integrator;
+
+enum nb_integrator_kind {
+    /* Kick-drift-kick leapfrog with the acceleration cached between steps
+     * (CUDA/src/integrator_leafrog.cu:47-103); bit-identical to the two-eval
+     * CPU form CPU/src/integrator_leapfrog.cpp:3-26 for deterministic forces. */
+    NB_LEAPFROG = 0,
+    /* Classic RK4 (CPU/src/integrator_rk4.cpp:3-47). */
+    NB_RK4 = 1
+};
+
+int nb_integrator_create(nb_integrator** it, int kind);
+int nb_integrator_destroy(nb_integrator* it);
+/* TimeIntegrator::step(ForceCalculator&, ParticleArrays&, double dt)
+ * (CUDA/include/integrator.hpp:10-14; CPU form CPU/include/integrator.hpp:16-20). */
+int nb_integrator_step(nb_integrator* it, nb_force* f, nb_particles* p, double dt);
+/* `steps` calls of nb_integrator_step with the kick of one step fused into the
+ * next step's kick-drift kernel; same bits as the step-by-step loop. */
+int nb_integrator_run(nb_integrator* it, nb_force* f, nb_particles* p, double dt, int steps);
+/* Forget the cached acceleration (next leapfrog step evaluates the force first). */
+int nb_integrator_reset(nb_integrator* it);
+/* TimeIntegrator::name() (CPU/include/integrator.hpp:22). */
+const char* nb_integrator_name(const nb_integrator* it);
+
+/* -------------------------------------------------------------- diagnostics */
+/* Energy / momentum diagnostics reduced on the GPU in FP64 with compensated
+ * (Neumaier) sums.  Replaces kineticEnergy / potentialEnergy / angularMomentum /
+ * centerOfMass (CPU/include/tests_common.hpp:33-75, CUDA/src/test_common.cpp:63-182).
+ * The potential is the O(N^2) pair sum; pe uses the reference's unsoftened
+ * 1/r (tests_common.hpp:47-49), pe_soft uses 1/sqrt(r^2+eps^2). */
+typedef struct nb_diag {
+    double ke, pe, pe_soft;
+    double p[3];   /* linear momentum */
+    double l[3];   /* angular momentum */
+    double com[3]; /* centre of mass */
+    double mass;
+} nb_diag;
+int nb_diagnostics(const nb_particles* p, double G, double softening, int with_potential,
+                   nb_diag* out);
+
+/* ------------------------------------------------- initial conditions (host) */
+/* Counter-based Philox4x32-10 (Salmon et al., SC'11) streams: identical on
+ * every host.  All arrays are caller-owned host arrays of length n. */
+/* ctr = {c0,c1,c2,c3}, key = {k0,k1} -> out[4]. */
+void nb_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Plummer sphere: scale radius a, total mass M (m_i = M/n), G, isotropic
+ * velocities from the distribution function (Aarseth, Henon & Wielen 1974);
+ * radii truncated at rmax*a (rmax <= 0: no truncation); centre-of-mass frame. */
+int nb_ic_plummer(int64_t n, uint64_t seed, double a, double M, double G, double rmax, double* m,
+                  double* x, double* y, double* z, double* vx, double* vy, double* vz);
+/* Uniform positions in [-1,1]^3, Gaussian velocities (sigma per axis), m = 1/n. */
+int nb_ic_uniform_gaussian(int64_t n, uint64_t seed, double sigma, double* m, double* x, double* y,
+                           double* z, double* vx, double* vy, double* vz);
+/* Plummer sphere (fraction 1-disk_frac of M, scale a) plus a cold exponential
+ * disk (disk_frac of M, scale length rd, sech^2 thickness zd) on circular
+ * orbits in the enclosed mass. */
+int nb_ic_plummer_disk(int64_t n, uint64_t seed, double a, double M, double G, double disk_frac,
+                       double rd, double zd, double* m, double* x, double* y, double* z,
+                       double* vx, double* vy, double* vz);
+
+/* ------------------------------------------------------ particle-file layout */
+/* Text layout of particles_*.txt: line 1 "N", then N lines "m x y z vx vy vz"
+ * (reader: CUDA/tests/test3_correctness.cu:16-38).  nb_file_read with NULL
+ * arrays only returns the count.  nb_file_write prints %.17g (exact round trip). */
+int nb_file_read(const char* path, int64_t* n, double* m, double* x, double* y, double* z,
+                 double* vx, double* vy, double* vz, int64_t capacity);
+int nb_file_write(const char* path, int64_t n, const double* m, const double* x, const double* y,
+                  const double* z, const double* vx, const double* vy, const double* vz);
+
+/* ------------------------------------------------------- batched ensembles */
+/* Many independent systems of uneven size advanced together: one persistent
+ * direct-sum launch per force evaluation over all systems, leapfrog kick/drift
+ * over all particles, step sequences captured in a HIP graph.  No reference
+ * counterpart (the reference advances one system per process). */
+typedef struct nb_ensemble nb_ensemble;
+/* offsets[k] = first particle of system k (offsets[n_systems] = total);
+ * G, softening, dt per system.  State arrays are host arrays of the total. */
+int nb_ensemble_create(nb_ensemble** e, int n_systems, const int64_t* offsets, const double* G,
+                       const double* softening, const double* dt);
+int nb_ensemble_destroy(nb_ensemble* e);
+int nb_ensemble_upload(nb_ensemble* e, const double* m, const double* x, const double* y,
+                       const double* z, const double* vx, const double* vy, const double* vz);
+int nb_ensemble_download(nb_ensemble* e, double* x, double* y, double* z, double* vx, double* vy,
+                         double* vz, double* ax, double* ay, double* az);
+/* Advance every system by `steps` leapfrog steps.  use_graph=1 replays a
+ * captured HIP graph of the step sequence. */
+int nb_ensemble_run(nb_ensemble* e, int steps, int use_graph);
+/* The same steps with one launch sequence per system (baseline for the
+ * batched path; same per-system arithmetic, hence the same bits). */
+int nb_ensemble_run_unbatched(nb_ensemble* e, int steps);
+/* Autotuner: time the tile shapes per size bucket on this device and store
+ * the choice (also written to `path` if not NULL).  nb_ensemble_tuning
+ * returns the per-bucket shape codes (bucket b covers N in [2^(b+5), 2^(b+6))). */
+int nb_ensemble_autotune(nb_ensemble* e, const char* path);
+int nb_ensemble_load_tuning(nb_ensemble* e, const char* path);
+int nb_ensemble_tuning(const nb_ensemble* e, int32_t* shape_codes, int n_buckets);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_AMD_H */

@@ -1,0 +1,1157 @@
+// nb_bh.hip -- Barnes-Hut pipeline on gfx950.
+//
+//   bbox      k_bbox_partial + k_bbox_final: FP64 min/max, root cube exactly as
+//             CPU/src/force_calculator_barnes_hut_optimized.cpp:105-124
+//             (centre = (min+max)/2, size = max extent + 1e-9).
+//   keys      k_keys: 63-bit octant-path keys, 21 levels of the reference's FP64
+//             subdivision replayed per particle (child centre = c +/- (s/2)/2,
+//             octant = x>=c | y>=c<<1 | z>=c<<2; same file :159-161,:219-227).
+//             Sorting the keys orders particles by the CPU tree's child order.
+//   sort      LSD radix sort of (key, index), 8 passes of 8 bits, stable.
+//   gather    sorted float4 (x, y, z, m).
+//   build     k_levels + k_level_scan + k_emit: the tree the reference's
+//             insertion build produces (a cell splits iff it holds more than
+//             leaf_capacity particles and its half size is >= 1e-9), built from
+//             the sorted keys without host synchronisation.  A cell at level l
+//             holding particle i has more than cap particles iff some window
+//             [j, j+cap] with i-cap <= j <= i shares the level-l prefix, so each
+//             particle's leaf level is 1 + max of the shared-prefix lengths of
+//             key pairs (j, j+cap) around it.  Nodes are numbered in level
+//             order (key order inside a level); children of a node are
+//             contiguous.  Empty octants are not materialised.
+//   up-sweep  k_upsweep per level (deepest first): mass, centre of mass and the
+//             traceless quadrupole in FP64 for every level; FP32 copies for
+//             the walk.
+//   walk      k_walk: one wave64 per group of 64 consecutive sorted targets.
+//             The wave opens cells breadth-first (8 parents x 8 children per
+//             step, one child per lane), accepting a cell for the whole group
+//             when size^2 < theta^2 * d^2 with d the distance from the cell's
+//             centre of mass to the group's bounding box (the reference
+//             criterion of force_calculator_barnes_hut_optimized.cpp:88,
+//             evaluated conservatively for all 64 targets at once).  Accepted
+//             cells and opened leaves' particles are staged in LDS and
+//             evaluated 64 at a time by all lanes (one target per lane).
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+#include "nb_internal.hpp"
+
+namespace nb {
+
+constexpr int kMaxDigits = 21;   // octant digits in a key
+constexpr int kLevels = 22;      // node levels 0..21
+constexpr int kBlk = 256;
+constexpr int kSortRows = 16;
+constexpr int kSortTile = kBlk * kSortRows;
+constexpr int kSortBins = 256;
+constexpr int kSortPasses = 8;
+constexpr int kWalkFcap = 8192;  // walk frontier capacity per block (opened cells per level)
+constexpr int kWalkBlocksPerCU = 16;
+constexpr int kUpsweepGrid = 512;
+constexpr int kGroup = 64;      // walk group: cells of at most 64 particles
+constexpr int kCols = kLevels + 1;  // per-block count columns: 22 levels + group starts
+
+struct RootInfo {
+    double center[3];
+    double size;
+    double minv[3], maxv[3];
+    int lmax;         // deepest level a node can have (21-digit keys and the 1e-9 split rule)
+    int key_limited;  // 1 if the split rule alone would allow deeper nodes than the keys encode
+};
+
+struct LevelInfo {
+    int start[kLevels + 1];
+    int count[kLevels];
+    int total;
+    int max_level;
+    int n_groups;
+};
+
+struct Counters {
+    unsigned long long pp, pc;
+    int max_frontier;
+    int overflow;   // bit 0: walk frontier, bit 1: node capacity, bit 2: malformed tree
+    int oversize;   // leaves at the key-depth limit with > cap particles
+    int n_leaves;
+    int next_group;  // work counter of the persistent walk
+};
+
+}  // namespace nb
+
+struct nb::BHWorkspace {
+    int64_t cap_n = 0, cap_nodes = 0;
+    int64_t last_n = 0;
+    int last_cap = 8;
+    int last_valid = 0;
+    DevBuf root, bbox_part, keys0, keys1, vals0, vals1, hist, spos;
+    DevBuf lvl_leaf, lvl_sh, blkcnt, blkoff, levels, leaf_of, counters;
+    DevBuf n_pstart, n_pend, n_child, n_cend, n_level, n_mom, w_mac, w_link, w_q0, w_q1;
+    DevBuf walk_scratch, gflag, gstart;
+    int walk_blocks = 0;
+};
+
+namespace nb {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    return (1ull << lane_id()) - 1ull;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Number of leading octant digits two 63-bit keys share (0..21).
+__device__ __forceinline__ int shared_levels(unsigned long long a, unsigned long long b) {
+    const unsigned long long d = a ^ b;
+    if (d == 0) return kMaxDigits;
+    return (__clzll((long long)d) - 1) / 3;
+}
+
+// Exclusive scan of `count` ints with one 1024-thread block.  Returns total.
+__device__ int block_scan_1024(const int* in, int* out, int count, int* sh_wave /*[16]*/) {
+    const int tid = threadIdx.x;
+    const int per = (count + 1023) / 1024;
+    const int b = tid * per;
+    const int e = min(b + per, count);
+    int local = 0;
+    for (int k = b; k < e; ++k) local += in[k];
+    const int incl = wave_incl_scan(local);
+    const int w = tid >> 6;
+    if (lane_id() == 63) sh_wave[w] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int k = 0; k < 16; ++k) {
+            const int t = sh_wave[k];
+            sh_wave[k] = run;
+            run += t;
+        }
+        sh_wave[16] = run;
+    }
+    __syncthreads();
+    int run = sh_wave[w] + incl - local;
+    for (int k = b; k < e; ++k) {
+        const int t = in[k];
+        out[k] = run;
+        run += t;
+    }
+    const int total = sh_wave[16];
+    __syncthreads();
+    return total;
+}
+
+// ------------------------------------------------------------------ counters
+__global__ void k_reset_counters(Counters* c) {
+    c->pp = 0;
+    c->pc = 0;
+    c->max_frontier = 0;
+    c->overflow = 0;
+    c->oversize = 0;
+    c->n_leaves = 0;
+    c->next_group = 0;
+}
+
+// ------------------------------------------------------------------ bbox
+__global__ __launch_bounds__(kBlk) void k_bbox_partial(int n, const double* __restrict__ x,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ z,
+                                                       double* __restrict__ part) {
+    __shared__ double sh[6][kBlk];
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    double mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int i = blockIdx.x * kBlk + threadIdx.x; i < n; i += gridDim.x * kBlk) {
+        const double p[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = p[a] < mn[a] ? p[a] : mn[a];
+            mx[a] = p[a] > mx[a] ? p[a] : mx[a];
+        }
+    }
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sh[a][t] = mn[a];
+        sh[3 + a][t] = mx[a];
+    }
+    __syncthreads();
+    for (int s = kBlk / 2; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                sh[a][t] = fmin(sh[a][t], sh[a][t + s]);
+                sh[3 + a][t] = fmax(sh[3 + a][t], sh[3 + a][t + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (t < 6) part[blockIdx.x * 6 + t] = sh[t][0];
+}
+
+__global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* __restrict__ part,
+                                                     RootInfo* __restrict__ root) {
+    __shared__ double sh[6][kBlk];
+    const int t = threadIdx.x;
+    double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int k = t; k < nparts; k += kBlk) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            v[a] = fmin(v[a], part[k * 6 + a]);
+            v[3 + a] = fmax(v[3 + a], part[k * 6 + 3 + a]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 6; ++a) sh[a][t] = v[a];
+    __syncthreads();
+    for (int s = kBlk / 2; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                sh[a][t] = fmin(sh[a][t], sh[a][t + s]);
+                sh[3 + a][t] = fmax(sh[3 + a][t], sh[3 + a][t + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        double size = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            root->minv[a] = sh[a][0];
+            root->maxv[a] = sh[3 + a][0];
+            root->center[a] = (sh[a][0] + sh[3 + a][0]) / 2;
+            const double ext = sh[3 + a][0] - sh[a][0];
+            size = a == 0 ? ext : (ext > size ? ext : size);
+        }
+        size += 1e-9;
+        root->size = size;
+        // A node of size s may split iff s/2 >= 1e-9
+        // (force_calculator_barnes_hut_optimized.cpp:208-211).
+        int l = 0;
+        double s = size;
+        while (l < kMaxDigits && s / 2.0 >= 1e-9) {
+            s = s / 2.0;
+            ++l;
+        }
+        root->lmax = l;
+        root->key_limited = (l == kMaxDigits && s / 2.0 >= 1e-9) ? 1 : 0;
+    }
+}
+
+// ------------------------------------------------------------------ keys
+__global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__ x,
+                                               const double* __restrict__ y,
+                                               const double* __restrict__ z,
+                                               const RootInfo* __restrict__ root,
+                                               unsigned long long* __restrict__ keys,
+                                               unsigned* __restrict__ vals) {
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    double cx = root->center[0], cy = root->center[1], cz = root->center[2];
+    double s = root->size;
+    const double px = x[i], py = y[i], pz = z[i];
+    unsigned long long k = 0;
+#pragma unroll
+    for (int l = 0; l < kMaxDigits; ++l) {
+        const double half = s / 2.0;
+        const double off = half / 2.0;
+        unsigned d = 0;
+        if (px >= cx) {
+            d |= 1u;
+            cx = cx + off;
+        } else {
+            cx = cx + (-off);
+        }
+        if (py >= cy) {
+            d |= 2u;
+            cy = cy + off;
+        } else {
+            cy = cy + (-off);
+        }
+        if (pz >= cz) {
+            d |= 4u;
+            cz = cz + off;
+        } else {
+            cz = cz + (-off);
+        }
+        k = (k << 3) | d;
+        s = half;
+    }
+    keys[i] = k;
+    vals[i] = (unsigned)i;
+}
+
+// ------------------------------------------------------------------ radix sort
+__global__ __launch_bounds__(kBlk) void k_radix_hist(int n, const unsigned long long* __restrict__ keys,
+                                                     int shift, int nblk, int* __restrict__ hist) {
+    __shared__ int h[kSortBins];
+    const int t = threadIdx.x;
+    h[t] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kSortRows; ++r) {
+        const int i = base + r * kBlk + t;
+        if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & 255ull)], 1);
+    }
+    __syncthreads();
+    hist[t * nblk + blockIdx.x] = h[t];
+}
+
+__global__ __launch_bounds__(1024) void k_radix_scan(int count, int* __restrict__ hist) {
+    __shared__ int shw[17];
+    block_scan_1024(hist, hist, count, shw);
+}
+
+// Stable scatter: items of a tile are ranked in (row, wave, lane) order, which
+// is their input order.  Lanes holding the same digit are found with eight
+// ballots; per-wave digit counts go through LDS.
+__global__ __launch_bounds__(kBlk) void k_radix_scatter(
+    int n, const unsigned long long* __restrict__ kin, const unsigned* __restrict__ vin,
+    unsigned long long* __restrict__ kout, unsigned* __restrict__ vout, int shift, int nblk,
+    const int* __restrict__ offs) {
+    __shared__ int run[kSortBins];
+    __shared__ int wc[4][kSortBins];
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    run[t] = offs[t * nblk + blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wc[k][t] = 0;
+    __syncthreads();
+    const unsigned long long lt = lanemask_lt();
+    const int base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRows; ++r) {
+        const int i = base + r * kBlk + t;
+        const bool valid = i < n;
+        const unsigned long long key = valid ? kin[i] : 0ull;
+        const unsigned val = valid ? vin[i] : 0u;
+        const int d = (int)((key >> shift) & 255ull);
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long bb = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? bb : ~bb;
+        }
+        const int rank = __popcll(peers & lt);
+        if (valid && rank == 0) wc[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int pos = run[d] + rank;
+            for (int k = 0; k < w; ++k) pos += wc[k][d];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wc[k][t] = 0;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlk) void k_gather(int n, const unsigned* __restrict__ perm,
+                                                 const double* __restrict__ x,
+                                                 const double* __restrict__ y,
+                                                 const double* __restrict__ z,
+                                                 const double* __restrict__ m,
+                                                 float4* __restrict__ spos) {
+    const int k = blockIdx.x * kBlk + threadIdx.x;
+    if (k >= n) return;
+    const unsigned j = perm[k];
+    spos[k] = make_float4((float)x[j], (float)y[j], (float)z[j], (float)m[j]);
+}
+
+// ------------------------------------------------------------------ build
+// Leaf level L_i and shared prefix with the predecessor sh_i of every sorted
+// particle, the walk-group start flags, and per-block counts of nodes starting
+// at each level (columns 0..21) and of group starts (column 22).
+//
+// A walk group is the tree cell of the smallest level that holds at most
+// kGroup particles (or the particle's leaf if that is shallower), so a group
+// is spatially compact; its particles are contiguous in sorted order.
+__global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long* __restrict__ keys,
+                                                 int cap, const RootInfo* __restrict__ root,
+                                                 signed char* __restrict__ Lout,
+                                                 signed char* __restrict__ shout,
+                                                 unsigned char* __restrict__ gflag, int nblk,
+                                                 int* __restrict__ blkcnt,
+                                                 Counters* __restrict__ ctr) {
+    constexpr int H = kGroup;  // key halo on both sides of the block
+    __shared__ unsigned long long sk[kBlk + 2 * H];
+    __shared__ signed char ss[kBlk + H];
+    __shared__ signed char sg[kBlk + H];
+    __shared__ int wcnt[4][kCols];
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    const int b0 = blockIdx.x * kBlk;
+    for (int q = t; q < kBlk + 2 * H; q += kBlk) {
+        const int g = b0 - H + q;
+        sk[q] = (g >= 0 && g < n) ? keys[g] : 0ull;
+    }
+    __syncthreads();
+    // ss[u]: shared levels of keys (j, j+cap) with j = b0 - cap + u;
+    // sg[u]: shared levels of keys (j, j+kGroup) with j = b0 - kGroup + u.
+    for (int u = t; u < kBlk + H; u += kBlk) {
+        if (u < kBlk + cap) {
+            const int j = b0 - cap + u;
+            ss[u] = (j >= 0 && j + cap < n)
+                        ? (signed char)shared_levels(sk[H - cap + u], sk[H + u])
+                        : (signed char)-1;
+        }
+        const int j = b0 - kGroup + u;
+        sg[u] = (j >= 0 && j + kGroup < n) ? (signed char)shared_levels(sk[u], sk[kGroup + u])
+                                           : (signed char)-1;
+    }
+    __syncthreads();
+    const int i = b0 + t;
+    const bool valid = i < n;
+    const int lmax = root->lmax;
+    int Li = -1, shi = 0, mx = -1, gl = -1;
+    if (valid) {
+        for (int q = 0; q <= cap; ++q) mx = max(mx, (int)ss[t + q]);
+        int mg = -1;
+        for (int q = 0; q <= kGroup; ++q) mg = max(mg, (int)sg[t + q]);
+        Li = min(lmax, mx + 1);
+        gl = min(Li, mg + 1);
+        shi = (i == 0) ? -1 : shared_levels(sk[H + t - 1], sk[H + t]);
+        Lout[i] = (signed char)Li;
+        shout[i] = (signed char)shi;
+    }
+    const bool leaf_start = valid && shi < Li;
+    const bool group_start = valid && shi < gl;
+    if (valid) gflag[i] = group_start ? 1 : 0;
+    const bool over = leaf_start && mx + 1 > lmax && root->key_limited;
+    const unsigned long long bl = __ballot(leaf_start), bo = __ballot(over);
+    if (lane_id() == 0) {
+        if (bl) atomicAdd(&ctr->n_leaves, __popcll(bl));
+        if (bo) atomicAdd(&ctr->oversize, __popcll(bo));
+    }
+#pragma unroll
+    for (int l = 0; l < kLevels; ++l) {
+        const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
+        if (lane_id() == 0) wcnt[w][l] = __popcll(bal);
+    }
+    {
+        const unsigned long long bal = __ballot(group_start);
+        if (lane_id() == 0) wcnt[w][kLevels] = __popcll(bal);
+    }
+    __syncthreads();
+    if (t < kCols)
+        blkcnt[t * nblk + blockIdx.x] = wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+}
+
+__global__ __launch_bounds__(1024) void k_level_scan(int n, int nblk, int* __restrict__ blk,
+                                                     LevelInfo* __restrict__ li, int node_cap,
+                                                     int* __restrict__ gstart,
+                                                     Counters* __restrict__ ctr) {
+    __shared__ int shw[17];
+    __shared__ int totals[kCols];
+    for (int l = 0; l < kCols; ++l) {
+        const int tot = block_scan_1024(blk + l * nblk, blk + l * nblk, nblk, shw);
+        if (threadIdx.x == 0) totals[l] = tot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0, ml = 0;
+        for (int l = 0; l < kLevels; ++l) {
+            li->start[l] = run;
+            li->count[l] = totals[l];
+            run += totals[l];
+            if (totals[l] > 0) ml = l;
+        }
+        li->start[kLevels] = run;
+        li->total = run;
+        li->max_level = ml;
+        li->n_groups = totals[kLevels];
+        gstart[totals[kLevels]] = n;
+        if (run > node_cap) atomicOr(&ctr->overflow, 2);
+    }
+}
+
+// Writes the node records: particle range, level, child range; and the leaf
+// of every sorted particle.
+__global__ __launch_bounds__(kBlk) void k_emit(
+    int n, const signed char* __restrict__ Lin, const signed char* __restrict__ shin, int nblk,
+    const int* __restrict__ blkoff, const LevelInfo* __restrict__ li, int node_cap,
+    int* __restrict__ pstart, int* __restrict__ pend, int* __restrict__ child,
+    int* __restrict__ cend, signed char* __restrict__ level, int* __restrict__ leaf_of,
+    const unsigned char* __restrict__ gflag, int* __restrict__ gstart,
+    Counters* __restrict__ ctr) {
+    __shared__ int wcnt[4][kCols];
+    __shared__ int lstart[kLevels + 1];
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    const int i = blockIdx.x * kBlk + t;
+    const bool valid = i < n;
+    const int Li = valid ? (int)Lin[i] : -1;
+    const int shi = valid ? (int)shin[i] : 0;
+    const int shn = (valid && i + 1 < n) ? (int)shin[i + 1] : -1;
+    if (t <= kLevels) lstart[t] = li->start[t];
+    const unsigned long long lt = lanemask_lt();
+    int rex[kLevels];
+#pragma unroll
+    for (int l = 0; l < kLevels; ++l) {
+        const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
+        rex[l] = __popcll(bal & lt);
+        if (lane_id() == 0) wcnt[w][l] = __popcll(bal);
+    }
+    const bool gs = valid && gflag[i];
+    int grank;
+    {
+        const unsigned long long bal = __ballot(gs);
+        grank = __popcll(bal & lt);
+        if (lane_id() == 0) wcnt[w][kLevels] = __popcll(bal);
+    }
+    __syncthreads();
+    if (gs) {
+        int base = blkoff[kLevels * nblk + blockIdx.x];
+        for (int k = 0; k < w; ++k) base += wcnt[k][kLevels];
+        gstart[base + grank] = i;
+    }
+#pragma unroll
+    for (int l = 0; l < kLevels; ++l) {
+        int base = blkoff[l * nblk + blockIdx.x];
+        for (int k = 0; k < w; ++k) base += wcnt[k][l];
+        rex[l] += base;
+    }
+    if (!valid || li->total > node_cap) return;
+#pragma unroll
+    for (int l = 0; l < kLevels; ++l) {
+        if (l <= Li) {
+            const bool starts = shi < l;
+            const int id = lstart[l] + rex[l] + (starts ? 1 : 0) - 1;
+            if (id < lstart[l] || id >= lstart[l + 1]) {
+                atomicOr(&ctr->overflow, 4);
+                continue;
+            }
+            if (starts) {
+                pstart[id] = i;
+                level[id] = (signed char)l;
+                child[id] = (l < Li) ? lstart[l + 1] + rex[min(l + 1, kLevels - 1)] : -1;
+            }
+            if (l > shn) {
+                pend[id] = i + 1;
+                cend[id] = (l < Li)
+                               ? lstart[l + 1] + rex[min(l + 1, kLevels - 1)] + (shi <= l ? 1 : 0)
+                               : -1;
+            }
+            if (l == Li) leaf_of[i] = id;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ up-sweep
+__global__ __launch_bounds__(kBlk) void k_upsweep(
+    int l, const LevelInfo* __restrict__ li, const RootInfo* __restrict__ root,
+    const int* __restrict__ pstart, const int* __restrict__ pend, const int* __restrict__ child,
+    const int* __restrict__ cend, const float4* __restrict__ spos, double* __restrict__ mom,
+    float4* __restrict__ mac, int2* __restrict__ link, float4* __restrict__ q0,
+    float4* __restrict__ q1, double inv_theta2, int quad, int n, Counters* __restrict__ ctr) {
+    const int cnt = li->count[l];
+    const int st = li->start[l];
+    const int total = li->total;
+    const double size = ldexp(root->size, -l);
+    const double r2open = size * size * inv_theta2;
+    for (int k = blockIdx.x * kBlk + threadIdx.x; k < cnt; k += gridDim.x * kBlk) {
+        const int id = st + k;
+        const int c0 = child[id];
+        // Structural checks: a malformed tree is reported, never dereferenced.
+        if (c0 < 0) {
+            const int a = pstart[id], b = pend[id];
+            if (a < 0 || b > n || a >= b) {
+                atomicOr(&ctr->overflow, 4);
+                link[id] = make_int2(~0, 0);
+                continue;
+            }
+        } else {
+            const int c1 = cend[id];
+            if (c0 <= id || c1 > total || c0 >= c1 || c1 - c0 > 8) {
+                atomicOr(&ctr->overflow, 4);
+                link[id] = make_int2(~0, 0);
+                continue;
+            }
+        }
+        double M = 0, cx = 0, cy = 0, cz = 0;
+        double Q[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz
+        if (c0 < 0) {
+            const int a = pstart[id], b = pend[id];
+            double sx = 0, sy = 0, sz = 0, ux = 0, uy = 0, uz = 0;
+            for (int q = a; q < b; ++q) {
+                const float4 p = spos[q];
+                const double m = p.w;
+                M += m;
+                sx += m * p.x;
+                sy += m * p.y;
+                sz += m * p.z;
+                ux += p.x;
+                uy += p.y;
+                uz += p.z;
+            }
+            if (M > 0) {
+                cx = sx / M;
+                cy = sy / M;
+                cz = sz / M;
+            } else {
+                const double inv = 1.0 / (double)(b - a);
+                cx = ux * inv;
+                cy = uy * inv;
+                cz = uz * inv;
+            }
+            if (quad) {
+                for (int q = a; q < b; ++q) {
+                    const float4 p = spos[q];
+                    const double m = p.w;
+                    const double dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    Q[0] += m * (3.0 * dx * dx - r2);
+                    Q[1] += m * (3.0 * dy * dy - r2);
+                    Q[2] += m * (3.0 * dz * dz - r2);
+                    Q[3] += m * (3.0 * dx * dy);
+                    Q[4] += m * (3.0 * dx * dz);
+                    Q[5] += m * (3.0 * dy * dz);
+                }
+            }
+        } else {
+            const int c1 = cend[id];
+            double sx = 0, sy = 0, sz = 0, ux = 0, uy = 0, uz = 0;
+            for (int c = c0; c < c1; ++c) {
+                const double* mc = mom + (size_t)c * 10;
+                M += mc[0];
+                sx += mc[0] * mc[1];
+                sy += mc[0] * mc[2];
+                sz += mc[0] * mc[3];
+                ux += mc[1];
+                uy += mc[2];
+                uz += mc[3];
+            }
+            if (M > 0) {
+                cx = sx / M;
+                cy = sy / M;
+                cz = sz / M;
+            } else {
+                const double inv = 1.0 / (double)(c1 - c0);
+                cx = ux * inv;
+                cy = uy * inv;
+                cz = uz * inv;
+            }
+            if (quad) {
+                for (int c = c0; c < c1; ++c) {
+                    const double* mc = mom + (size_t)c * 10;
+                    const double m = mc[0];
+                    const double dx = mc[1] - cx, dy = mc[2] - cy, dz = mc[3] - cz;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    Q[0] += mc[4] + m * (3.0 * dx * dx - r2);
+                    Q[1] += mc[5] + m * (3.0 * dy * dy - r2);
+                    Q[2] += mc[6] + m * (3.0 * dz * dz - r2);
+                    Q[3] += mc[7] + m * (3.0 * dx * dy);
+                    Q[4] += mc[8] + m * (3.0 * dx * dz);
+                    Q[5] += mc[9] + m * (3.0 * dy * dz);
+                }
+            }
+        }
+        double* mo = mom + (size_t)id * 10;
+        mo[0] = M;
+        mo[1] = cx;
+        mo[2] = cy;
+        mo[3] = cz;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) mo[4 + a] = Q[a];
+        mac[id] = make_float4((float)cx, (float)cy, (float)cz, (float)r2open);
+        link[id] = c0 < 0 ? make_int2(~pstart[id], pend[id]) : make_int2(c0, cend[id]);
+        q0[id] = make_float4((float)M, (float)Q[0], (float)Q[1], (float)Q[2]);
+        q1[id] = make_float4((float)Q[3], (float)Q[4], (float)Q[5], 0.f);
+    }
+}
+
+// ------------------------------------------------------------------ walk
+// Persistent: each one-wave block takes groups of 64 sorted targets from an
+// atomic counter.  The breadth-first frontier holds the child ranges of
+// opened cells and lives in a per-block slice of global scratch (it can reach
+// thousands of entries in dense regions); interaction lists live in LDS.
+template <bool kQuad, bool kEps0>
+__global__ __launch_bounds__(64) void k_walk(int n, const float4* __restrict__ spos,
+                                             const unsigned* __restrict__ perm,
+                                             const float4* __restrict__ mac,
+                                             const int2* __restrict__ link,
+                                             const float4* __restrict__ q0,
+                                             const float4* __restrict__ q1, float soft2,
+                                             double G, double* __restrict__ ax,
+                                             double* __restrict__ ay, double* __restrict__ az,
+                                             int2* __restrict__ scratch, int fcap,
+                                             const int* __restrict__ gstart,
+                                             const LevelInfo* __restrict__ li,
+                                             Counters* __restrict__ ctr) {
+    // A tree that failed the build/up-sweep checks is not walked.
+    if (ctr->overflow & 6) return;
+    __shared__ int cl[128];
+    __shared__ float4 pl[128];
+    __shared__ int eofs[64];
+    __shared__ int ebeg[64];
+    __shared__ float4 st0[64], st1[64], st2[64];
+    __shared__ int sgroup;
+
+    const int lane = threadIdx.x;
+    const unsigned long long lt = lanemask_lt();
+    int2* fr0 = scratch + (size_t)blockIdx.x * 2 * fcap;
+    int2* fr1 = fr0 + fcap;
+    const int ngroups = li->n_groups;
+    unsigned long long tot_pp = 0, tot_pc = 0;
+    int maxf = 0;
+    bool ovf = false;
+
+    for (;;) {
+        if (lane == 0) sgroup = atomicAdd(&ctr->next_group, 1);
+        __syncthreads();
+        const int grp = sgroup;
+        __syncthreads();
+        if (grp >= ngroups) break;
+        const int gend = gstart[grp + 1];
+      for (int g0 = gstart[grp]; g0 < gend; g0 += 64) {  // groups above 64 (oversize leaves) in chunks
+        const int k = g0 + lane;
+        const bool active = k < gend;
+        const int nact = min(64, gend - g0);
+        const float4 me = spos[active ? k : g0];
+
+        // Group bounding box.
+        float lo0 = me.x, lo1 = me.y, lo2 = me.z, hi0 = me.x, hi1 = me.y, hi2 = me.z;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            lo0 = fminf(lo0, __shfl_xor(lo0, d, 64));
+            lo1 = fminf(lo1, __shfl_xor(lo1, d, 64));
+            lo2 = fminf(lo2, __shfl_xor(lo2, d, 64));
+            hi0 = fmaxf(hi0, __shfl_xor(hi0, d, 64));
+            hi1 = fmaxf(hi1, __shfl_xor(hi1, d, 64));
+            hi2 = fmaxf(hi2, __shfl_xor(hi2, d, 64));
+        }
+        const float bc0 = 0.5f * (lo0 + hi0), bc1 = 0.5f * (lo1 + hi1), bc2 = 0.5f * (lo2 + hi2);
+        const float bh0 = fmaxf(hi0 - bc0, bc0 - lo0), bh1 = fmaxf(hi1 - bc1, bc1 - lo1),
+                    bh2 = fmaxf(hi2 - bc2, bc2 - lo2);
+
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        int ncl = 0, npl = 0, nnext = 0;
+        unsigned long long npp = 0, npc = 0;
+        int2* nxt = fr0;
+
+        auto flush_particles = [&](int cnt) {
+            for (int j = 0; j < cnt; ++j) {
+                const float4 s = pl[j];
+                const float dx = s.x - me.x, dy = s.y - me.y, dz = s.z - me.z;
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
+                float inv = __builtin_amdgcn_rsqf(r2);
+                if (kEps0) inv = r2 > 0.f ? inv : 0.f;
+                const float c = (s.w * inv) * (inv * inv);
+                fx = fmaf(dx, c, fx);
+                fy = fmaf(dy, c, fy);
+                fz = fmaf(dz, c, fz);
+            }
+            npp += (unsigned long long)cnt;
+        };
+
+        auto flush_cells = [&](int cnt) {
+            if (lane < cnt) {
+                const int id = cl[lane];
+                st0[lane] = mac[id];
+                st1[lane] = q0[id];
+                if (kQuad) st2[lane] = q1[id];
+            }
+            __syncthreads();
+            for (int j = 0; j < cnt; ++j) {
+                const float4 c = st0[j];
+                const float4 qa = st1[j];
+                const float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float inv2 = inv * inv;
+                const float inv3 = inv * inv2;
+                if (kQuad) {
+                    const float4 qb = st2[j];
+                    // Q = [[xx xy xz],[xy yy yz],[xz yz zz]]: qa = (M, xx, yy, zz), qb = (xy, xz, yz)
+                    const float qdx = fmaf(qa.y, dx, fmaf(qb.x, dy, qb.y * dz));
+                    const float qdy = fmaf(qb.x, dx, fmaf(qa.z, dy, qb.z * dz));
+                    const float qdz = fmaf(qb.y, dx, fmaf(qb.z, dy, qa.w * dz));
+                    const float dqd = fmaf(dx, qdx, fmaf(dy, qdy, dz * qdz));
+                    const float inv5 = inv3 * inv2;
+                    const float inv7 = inv5 * inv2;
+                    const float cm = fmaf(2.5f * dqd, inv7, qa.x * inv3);
+                    fx = fmaf(dx, cm, fmaf(-qdx, inv5, fx));
+                    fy = fmaf(dy, cm, fmaf(-qdy, inv5, fy));
+                    fz = fmaf(dz, cm, fmaf(-qdz, inv5, fz));
+                } else {
+                    const float cm = qa.x * inv3;
+                    fx = fmaf(dx, cm, fx);
+                    fy = fmaf(dy, cm, fy);
+                    fz = fmaf(dz, cm, fz);
+                }
+            }
+            npc += (unsigned long long)cnt;
+            __syncthreads();
+        };
+
+        // Classifies one candidate node per lane (id < 0: none) against the group.
+        auto process = [&](int id) {
+            const bool valid = id >= 0;
+            float4 mc = make_float4(0.f, 0.f, 0.f, 0.f);
+            int2 lk = make_int2(0, 0);
+            if (valid) {
+                mc = mac[id];
+                lk = link[id];
+            }
+            const float e0 = fmaxf(fabsf(mc.x - bc0) - bh0, 0.f);
+            const float e1 = fmaxf(fabsf(mc.y - bc1) - bh1, 0.f);
+            const float e2 = fmaxf(fabsf(mc.z - bc2) - bh2, 0.f);
+            const float d2 = fmaf(e0, e0, fmaf(e1, e1, e2 * e2));
+            const bool acc = valid && d2 > mc.w;
+            const bool leaf = valid && !acc && lk.x < 0;
+            const bool open = valid && !acc && lk.x >= 0;
+
+            const unsigned long long bo = __ballot(open);
+            const int no = __popcll(bo);
+            if (open) {
+                const int pos = nnext + __popcll(bo & lt);
+                if (pos < fcap) nxt[pos] = lk;
+            }
+            if (nnext + no > fcap) ovf = true;
+            nnext = min(nnext + no, fcap);
+
+            const unsigned long long ba = __ballot(acc);
+            if (acc) cl[ncl + __popcll(ba & lt)] = id;
+            ncl += __popcll(ba);
+            __syncthreads();
+            if (ncl >= 64) {
+                flush_cells(64);
+                const bool mv = lane < ncl - 64;
+                int v = 0;
+                if (mv) v = cl[64 + lane];
+                __syncthreads();
+                if (mv) cl[lane] = v;
+                ncl -= 64;
+                __syncthreads();
+            }
+
+            const int cnt = leaf ? (lk.y - ~lk.x) : 0;
+            const int incl = wave_incl_scan(cnt);
+            const int tot = __shfl(incl, 63, 64);
+            if (tot > 0) {
+                eofs[lane] = incl - cnt;
+                ebeg[lane] = leaf ? ~lk.x : 0;
+                __syncthreads();
+                for (int t0 = 0; t0 < tot; t0 += 64) {
+                    const int q = t0 + lane;
+                    if (q < tot) {
+                        int lo = 0, hi = 64;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (eofs[mid] <= q) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        const int own = lo - 1;
+                        pl[npl + lane] = spos[ebeg[own] + (q - eofs[own])];
+                    }
+                    npl += min(64, tot - t0);
+                    __syncthreads();
+                    if (npl >= 64) {
+                        flush_particles(64);
+                        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                        const bool mv = lane < npl - 64;
+                        if (mv) v = pl[64 + lane];
+                        __syncthreads();
+                        if (mv) pl[lane] = v;
+                        npl -= 64;
+                        __syncthreads();
+                    }
+                }
+            }
+        };
+
+        // Root (node 0), then one tree level per pass.
+        process(lane == 0 ? 0 : -1);
+        int2* cur = fr0;
+        int ncur = nnext;
+        while (ncur > 0) {
+            maxf = max(maxf, ncur);
+            nxt = cur == fr0 ? fr1 : fr0;
+            nnext = 0;
+            for (int base = 0; base < ncur; base += 8) {
+                const int ps = base + (lane >> 3);
+                const int cs = lane & 7;
+                int id = -1;
+                if (ps < ncur) {
+                    const int2 rg = cur[ps];
+                    if (cs < rg.y - rg.x) id = rg.x + cs;
+                }
+                process(id);
+            }
+            __syncthreads();
+            cur = nxt;
+            ncur = nnext;
+        }
+        if (ncl > 0) flush_cells(ncl);
+        if (npl > 0) flush_particles(npl);
+
+        if (active) {
+            const unsigned j = perm[k];
+            ax[j] = G * (double)fx;
+            ay[j] = G * (double)fy;
+            az[j] = G * (double)fz;
+        }
+        tot_pp += npp * (unsigned long long)nact;
+        tot_pc += npc * (unsigned long long)nact;
+      }
+    }
+    if (lane == 0) {
+        atomicAdd(&ctr->pp, tot_pp);
+        atomicAdd(&ctr->pc, tot_pc);
+        atomicMax(&ctr->max_frontier, maxf);
+        if (ovf) atomicOr(&ctr->overflow, 1);
+    }
+}
+
+// ------------------------------------------------------------------ host
+int bh_create(BHWorkspace** w) {
+    *w = new BHWorkspace();
+    return NB_OK;
+}
+
+void bh_destroy(BHWorkspace* w) { delete w; }
+
+static int64_t node_capacity(int64_t n, int cap) {
+    return n + (int64_t)kMaxDigits * ((n + cap) / (cap + 1)) + 1;
+}
+
+int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, int multipole,
+            int64_t n64, const double* m, const double* x, const double* y, const double* z,
+            double* ax, double* ay, double* az, hipStream_t s, hipEvent_t ev_build_done) {
+    if (n64 <= 0) return NB_OK;
+    if (n64 > (int64_t)1 << 28) {
+        set_error("Barnes-Hut: n=%lld exceeds 2^28", (long long)n64);
+        return NB_ERR_INVALID;
+    }
+    const int n = (int)n64;
+    const int nblk = grid_for(n, kBlk);
+    const int nsort = grid_for(n, kSortTile);
+    const int nbb = std::min(1024, nblk);
+    const int64_t ncap = node_capacity(n, leaf_cap);
+    if (ncap > INT_MAX / 2) {
+        set_error("Barnes-Hut: node capacity overflow");
+        return NB_ERR_INVALID;
+    }
+
+    NB_TRY(w->root.reserve(sizeof(RootInfo)));
+    NB_TRY(w->levels.reserve(sizeof(LevelInfo)));
+    NB_TRY(w->counters.reserve(sizeof(Counters)));
+    NB_TRY(w->bbox_part.reserve((size_t)nbb * 6 * sizeof(double)));
+    NB_TRY(w->keys0.reserve((size_t)n * 8));
+    NB_TRY(w->keys1.reserve((size_t)n * 8));
+    NB_TRY(w->vals0.reserve((size_t)n * 4));
+    NB_TRY(w->vals1.reserve((size_t)n * 4));
+    NB_TRY(w->hist.reserve((size_t)nsort * kSortBins * 4));
+    NB_TRY(w->spos.reserve((size_t)n * 16));
+    NB_TRY(w->lvl_leaf.reserve((size_t)n));
+    NB_TRY(w->lvl_sh.reserve((size_t)n));
+    NB_TRY(w->blkcnt.reserve((size_t)nblk * kCols * 4));
+    NB_TRY(w->gflag.reserve((size_t)n));
+    NB_TRY(w->gstart.reserve((size_t)(n + 1) * 4));
+    NB_TRY(w->leaf_of.reserve((size_t)n * 4));
+    NB_TRY(w->n_pstart.reserve((size_t)ncap * 4));
+    NB_TRY(w->n_pend.reserve((size_t)ncap * 4));
+    NB_TRY(w->n_child.reserve((size_t)ncap * 4));
+    NB_TRY(w->n_cend.reserve((size_t)ncap * 4));
+    NB_TRY(w->n_level.reserve((size_t)ncap));
+    NB_TRY(w->n_mom.reserve((size_t)ncap * 10 * 8));
+    NB_TRY(w->w_mac.reserve((size_t)ncap * 16));
+    NB_TRY(w->w_link.reserve((size_t)ncap * 8));
+    NB_TRY(w->w_q0.reserve((size_t)ncap * 16));
+    NB_TRY(w->w_q1.reserve((size_t)ncap * 16));
+
+    RootInfo* root = w->root.as<RootInfo>();
+    LevelInfo* li = w->levels.as<LevelInfo>();
+    Counters* ctr = w->counters.as<Counters>();
+    auto* k0 = w->keys0.as<unsigned long long>();
+    auto* k1 = w->keys1.as<unsigned long long>();
+    auto* v0 = w->vals0.as<unsigned>();
+    auto* v1 = w->vals1.as<unsigned>();
+
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, ctr);
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(kBlk), 0, s, n, x, y, z,
+                       w->bbox_part.as<double>());
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlk), 0, s, nbb, w->bbox_part.as<double>(),
+                       root);
+    hipLaunchKernelGGL(k_keys, dim3(nblk), dim3(kBlk), 0, s, n, x, y, z, root, k0, v0);
+    NB_LAUNCH_CHECK();
+    for (int p = 0; p < kSortPasses; ++p) {
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nsort), dim3(kBlk), 0, s, n, k0, shift, nsort,
+                           w->hist.as<int>());
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, nsort * kSortBins,
+                           w->hist.as<int>());
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nsort), dim3(kBlk), 0, s, n, k0, v0, k1, v1,
+                           shift, nsort, w->hist.as<int>());
+        std::swap(k0, k1);
+        std::swap(v0, v1);
+    }
+    NB_LAUNCH_CHECK();
+    // kSortPasses is even: the sorted keys are back in keys0/vals0.
+    hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlk), 0, s, n, v0, x, y, z, m,
+                       w->spos.as<float4>());
+    hipLaunchKernelGGL(k_levels, dim3(nblk), dim3(kBlk), 0, s, n, k0, leaf_cap, root,
+                       w->lvl_leaf.as<signed char>(), w->lvl_sh.as<signed char>(),
+                       w->gflag.as<unsigned char>(), nblk, w->blkcnt.as<int>(), ctr);
+    hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(1024), 0, s, n, nblk, w->blkcnt.as<int>(), li,
+                       (int)ncap, w->gstart.as<int>(), ctr);
+    hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(kBlk), 0, s, n, w->lvl_leaf.as<signed char>(),
+                       w->lvl_sh.as<signed char>(), nblk, w->blkcnt.as<int>(), li, (int)ncap,
+                       w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
+                       w->n_cend.as<int>(), w->n_level.as<signed char>(), w->leaf_of.as<int>(),
+                       w->gflag.as<unsigned char>(), w->gstart.as<int>(), ctr);
+    NB_LAUNCH_CHECK();
+    const double inv_theta2 = theta > 0.0 ? 1.0 / (theta * theta) : INFINITY;
+    for (int l = kLevels - 1; l >= 0; --l) {
+        hipLaunchKernelGGL(k_upsweep, dim3(kUpsweepGrid), dim3(kBlk), 0, s, l, li, root,
+                           w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
+                           w->n_cend.as<int>(), w->spos.as<float4>(), w->n_mom.as<double>(),
+                           w->w_mac.as<float4>(), w->w_link.as<int2>(), w->w_q0.as<float4>(),
+                           w->w_q1.as<float4>(), inv_theta2, multipole == NB_QUADRUPOLE ? 1 : 0,
+                           n, ctr);
+    }
+    NB_LAUNCH_CHECK();
+    if (ev_build_done) NB_HIP(hipEventRecord(ev_build_done, s));
+
+    const int ngroups = grid_for(n, 16);  // upper bound on walk groups in flight
+    if (w->walk_blocks == 0) {
+        int dev = 0, cus = 0;
+        NB_HIP(hipGetDevice(&dev));
+        NB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        w->walk_blocks = std::max(1, cus) * kWalkBlocksPerCU;
+    }
+    const int wblocks = std::min(ngroups, w->walk_blocks);
+    NB_TRY(w->walk_scratch.reserve((size_t)w->walk_blocks * 2 * kWalkFcap * sizeof(int2)));
+    const float s2 = (float)soft2;
+    const bool quad = multipole == NB_QUADRUPOLE;
+#define NB_WALK(Q, E)                                                                          \
+    hipLaunchKernelGGL((k_walk<Q, E>), dim3(wblocks), dim3(64), 0, s, n, w->spos.as<float4>(), \
+                       v0, w->w_mac.as<float4>(), w->w_link.as<int2>(), w->w_q0.as<float4>(),  \
+                       w->w_q1.as<float4>(), s2, G, ax, ay, az,                                 \
+                       w->walk_scratch.as<int2>(), kWalkFcap, w->gstart.as<int>(), li, ctr)
+    if (quad) {
+        if (s2 > 0.f) NB_WALK(true, false);
+        else NB_WALK(true, true);
+    } else {
+        if (s2 > 0.f) NB_WALK(false, false);
+        else NB_WALK(false, true);
+    }
+#undef NB_WALK
+    NB_LAUNCH_CHECK();
+    w->last_n = n;
+    w->last_cap = leaf_cap;
+    w->last_valid = 1;
+    return NB_OK;
+}
+
+int bh_stats(BHWorkspace* w, hipStream_t s, nb_bh_stats* out) {
+    if (!w->last_valid) {
+        set_error("Barnes-Hut stats: no evaluation yet");
+        return NB_ERR_STATE;
+    }
+    NB_HIP(hipStreamSynchronize(s));
+    RootInfo r;
+    LevelInfo li;
+    Counters c;
+    NB_HIP(hipMemcpy(&r, w->root.ptr, sizeof(r), hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(&li, w->levels.ptr, sizeof(li), hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(&c, w->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
+    out->n_nodes = li.total;
+    out->n_leaves = c.n_leaves;
+    out->max_level = li.max_level;
+    out->level_limit = r.lmax;
+    out->oversize_cells = c.oversize;
+    out->pp_interactions = (int64_t)c.pp;
+    out->pc_interactions = (int64_t)c.pc;
+    out->max_frontier = c.max_frontier;
+    out->frontier_overflow = c.overflow;
+    for (int a = 0; a < 3; ++a) out->root_center[a] = r.center[a];
+    out->root_size = r.size;
+    out->n_groups = li.n_groups;
+    if (c.overflow) {
+        set_error("Barnes-Hut: capacity exceeded or malformed tree (flags %d: 1=walk frontier, "
+                  "2=node pool, 4=tree consistency)",
+                  c.overflow);
+        return NB_ERR_CAPACITY;
+    }
+    return NB_OK;
+}
+
+int bh_topology(BHWorkspace* w, hipStream_t s, int64_t n, int32_t* depth, uint64_t* path,
+                uint64_t* keys, uint32_t* perm) {
+    if (!w->last_valid || n != w->last_n) {
+        set_error("Barnes-Hut topology: n=%lld does not match the last evaluation (%lld)",
+                  (long long)n, (long long)w->last_n);
+        return NB_ERR_STATE;
+    }
+    NB_HIP(hipStreamSynchronize(s));
+    LevelInfo li;
+    NB_HIP(hipMemcpy(&li, w->levels.ptr, sizeof(li), hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> k(n);
+    std::vector<unsigned> p(n);
+    std::vector<int> lf(n);
+    std::vector<signed char> lv(li.total);
+    NB_HIP(hipMemcpy(k.data(), w->keys0.ptr, n * 8, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(p.data(), w->vals0.ptr, n * 4, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(lf.data(), w->leaf_of.ptr, n * 4, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(lv.data(), w->n_level.ptr, li.total, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < n; ++q) {
+        const unsigned o = p[q];
+        const int L = lv[lf[q]];
+        if (depth) depth[o] = L;
+        if (path) path[o] = L == 0 ? 0ull : (k[q] >> (63 - 3 * L));
+        if (keys) keys[o] = k[q];
+        if (perm) perm[q] = o;
+    }
+    return NB_OK;
+}
+
+int bh_nodes(BHWorkspace* w, hipStream_t s, int64_t capacity, int64_t* count, int32_t* level,
+             uint64_t* path, int32_t* first, int32_t* last, double* mass, double* com,
+             double* quad) {
+    if (!w->last_valid) {
+        set_error("Barnes-Hut nodes: no evaluation yet");
+        return NB_ERR_STATE;
+    }
+    NB_HIP(hipStreamSynchronize(s));
+    LevelInfo li;
+    NB_HIP(hipMemcpy(&li, w->levels.ptr, sizeof(li), hipMemcpyDeviceToHost));
+    *count = li.total;
+    if (capacity < li.total) return NB_OK;  // caller asked for the count only
+    const int64_t nn = li.total;
+    std::vector<signed char> lv(nn);
+    std::vector<int> ps(nn), pe(nn);
+    std::vector<double> mo((size_t)nn * 10);
+    std::vector<unsigned long long> k(w->last_n);
+    NB_HIP(hipMemcpy(lv.data(), w->n_level.ptr, nn, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(ps.data(), w->n_pstart.ptr, nn * 4, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(pe.data(), w->n_pend.ptr, nn * 4, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(mo.data(), w->n_mom.ptr, nn * 80, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(k.data(), w->keys0.ptr, w->last_n * 8, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < nn; ++q) {
+        const int L = lv[q];
+        if (level) level[q] = L;
+        if (path) path[q] = L == 0 ? 0ull : (k[ps[q]] >> (63 - 3 * L));
+        if (first) first[q] = ps[q];
+        if (last) last[q] = pe[q];
+        if (mass) mass[q] = mo[q * 10];
+        if (com)
+            for (int a = 0; a < 3; ++a) com[q * 3 + a] = mo[q * 10 + 1 + a];
+        if (quad)
+            for (int a = 0; a < 6; ++a) quad[q * 6 + a] = mo[q * 10 + 4 + a];
+    }
+    return NB_OK;
+}
+
+}  // namespace nb

@@ -1,0 +1,308 @@
+// nb_runtime.hip -- error reporting, device selection, particle state and the
+// force-calculator handles of the C ABI.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+#include "nb_internal.hpp"
+
+namespace nb {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+void clear_error() { g_err[0] = 0; }
+
+int DevBuf::reserve(size_t want) {
+    if (want <= bytes && ptr) return NB_OK;
+    release();
+    if (want == 0) return NB_OK;
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        set_error("hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? NB_ERR_NOMEM : NB_ERR_HIP;
+    }
+    bytes = want;
+    return NB_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int force_eval(nb_force* f, int64_t n, const double* m, const double* x, const double* y,
+               const double* z, double* ax, double* ay, double* az) {
+    if (n <= 0) return NB_OK;
+    const bool timed = f->timing;
+    if (timed) NB_HIP(hipEventRecord(f->ev[0], f->stream));
+    int st;
+    if (f->kind == 0) {
+        st = direct_eval(f->precision, f->G, f->soft2, n, m, x, y, z, ax, ay, az, f->direct,
+                         f->stream);
+        if (st == NB_OK && timed) NB_HIP(hipEventRecord(f->ev[1], f->stream));
+    } else {
+        st = bh_eval(f->bh, f->G, f->soft2, f->theta, f->leaf_cap, f->multipole, n, m, x, y, z,
+                     ax, ay, az, f->stream, timed ? f->ev[1] : nullptr);
+    }
+    if (st != NB_OK) return st;
+    if (timed) {
+        NB_HIP(hipEventRecord(f->ev[2], f->stream));
+        NB_HIP(hipEventSynchronize(f->ev[2]));
+        float a = 0, b = 0;
+        NB_HIP(hipEventElapsedTime(&a, f->ev[0], f->ev[1]));
+        NB_HIP(hipEventElapsedTime(&b, f->ev[1], f->ev[2]));
+        if (f->kind == 0) {
+            f->t_walk += a;  // the whole direct sum is the "walk" phase
+        } else {
+            f->t_build += a;
+            f->t_walk += b;
+        }
+        f->t_total += a + b;
+    }
+    f->calls++;
+    return NB_OK;
+}
+
+}  // namespace nb
+
+using namespace nb;
+
+extern "C" {
+
+const char* nb_last_error(void) { return nb::g_err; }
+
+const char* nb_version(void) { return "nbody_amd 0.1 (gfx950)"; }
+
+int nb_device_count(int* count) {
+    if (!count) {
+        set_error("nb_device_count: null pointer");
+        return NB_ERR_INVALID;
+    }
+    NB_HIP(hipGetDeviceCount(count));
+    return NB_OK;
+}
+
+int nb_set_device(int device) {
+    NB_HIP(hipSetDevice(device));
+    return NB_OK;
+}
+
+int nb_synchronize(void) {
+    NB_HIP(hipDeviceSynchronize());
+    return NB_OK;
+}
+
+// ------------------------------------------------------------------ particles
+int nb_particles_alloc(nb_particles* p, int64_t n) {
+    if (!p || n < 0) {
+        set_error("nb_particles_alloc: invalid arguments (n=%lld)", (long long)n);
+        return NB_ERR_INVALID;
+    }
+    std::memset(p, 0, sizeof(*p));
+    p->n = n;
+    if (n == 0) return NB_OK;
+    double** arrs[10] = {&p->x, &p->y, &p->z, &p->vx, &p->vy, &p->vz, &p->ax, &p->ay, &p->az, &p->m};
+    // One allocation, ten 256-byte-aligned arrays.
+    const size_t stride = ((size_t)n * sizeof(double) + 255) & ~(size_t)255;
+    char* base = nullptr;
+    hipError_t e = hipMalloc((void**)&base, stride * 10);
+    if (e != hipSuccess) {
+        set_error("nb_particles_alloc: hipMalloc(%zu) failed: %s", stride * 10, hipGetErrorString(e));
+        p->n = 0;
+        return e == hipErrorOutOfMemory ? NB_ERR_NOMEM : NB_ERR_HIP;
+    }
+    for (int k = 0; k < 10; ++k) *arrs[k] = reinterpret_cast<double*>(base + k * stride);
+    NB_HIP(hipMemset(base, 0, stride * 10));
+    return NB_OK;
+}
+
+int nb_particles_free(nb_particles* p) {
+    if (!p) return NB_OK;
+    if (p->x) NB_HIP(hipFree(p->x));
+    std::memset(p, 0, sizeof(*p));
+    return NB_OK;
+}
+
+int nb_particles_upload(nb_particles* p, const double* x, const double* y, const double* z,
+                        const double* vx, const double* vy, const double* vz, const double* m) {
+    if (!p || (p->n > 0 && (!x || !y || !z || !m))) {
+        set_error("nb_particles_upload: null argument");
+        return NB_ERR_INVALID;
+    }
+    const size_t b = (size_t)p->n * sizeof(double);
+    if (b == 0) return NB_OK;
+    NB_HIP(hipDeviceSynchronize());
+    NB_HIP(hipMemcpy(p->x, x, b, hipMemcpyHostToDevice));
+    NB_HIP(hipMemcpy(p->y, y, b, hipMemcpyHostToDevice));
+    NB_HIP(hipMemcpy(p->z, z, b, hipMemcpyHostToDevice));
+    NB_HIP(hipMemcpy(p->m, m, b, hipMemcpyHostToDevice));
+    if (vx && vy && vz) {
+        NB_HIP(hipMemcpy(p->vx, vx, b, hipMemcpyHostToDevice));
+        NB_HIP(hipMemcpy(p->vy, vy, b, hipMemcpyHostToDevice));
+        NB_HIP(hipMemcpy(p->vz, vz, b, hipMemcpyHostToDevice));
+    } else {
+        NB_HIP(hipMemset(p->vx, 0, b));
+        NB_HIP(hipMemset(p->vy, 0, b));
+        NB_HIP(hipMemset(p->vz, 0, b));
+    }
+    NB_HIP(hipDeviceSynchronize());
+    return NB_OK;
+}
+
+int nb_particles_download(const nb_particles* p, double* x, double* y, double* z, double* vx,
+                          double* vy, double* vz, double* ax, double* ay, double* az, double* m) {
+    if (!p) {
+        set_error("nb_particles_download: null particles");
+        return NB_ERR_INVALID;
+    }
+    const size_t b = (size_t)p->n * sizeof(double);
+    NB_HIP(hipDeviceSynchronize());
+    if (b == 0) return NB_OK;
+    const double* src[10] = {p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay, p->az, p->m};
+    double* dst[10] = {x, y, z, vx, vy, vz, ax, ay, az, m};
+    for (int k = 0; k < 10; ++k)
+        if (dst[k]) NB_HIP(hipMemcpy(dst[k], src[k], b, hipMemcpyDeviceToHost));
+    return NB_OK;
+}
+
+// ------------------------------------------------------------ force handles
+static int force_common_init(nb_force* f) {
+    NB_HIP(hipGetDevice(&f->device));
+    NB_HIP(hipStreamCreate(&f->stream));
+    for (auto& e : f->ev) NB_HIP(hipEventCreate(&e));
+    return NB_OK;
+}
+
+int nb_force_create_direct(nb_force** out, int precision, double G, double softening) {
+    if (!out || precision < NB_DIRECT_F64_EXACT || precision > NB_DIRECT_F32 || !std::isfinite(G) ||
+        !std::isfinite(softening)) {
+        set_error("nb_force_create_direct: invalid arguments");
+        return NB_ERR_INVALID;
+    }
+    nb_force* f = new nb_force();
+    f->kind = 0;
+    f->precision = precision;
+    f->G = G;
+    f->softening = softening;
+    f->soft2 = softening * softening;
+    f->name = precision == NB_DIRECT_F64_EXACT ? "Naive-F64-exact"
+              : precision == NB_DIRECT_F64     ? "Naive-F64"
+                                               : "Naive-F32-tiled";
+    int st = force_common_init(f);
+    if (st != NB_OK) {
+        nb_force_destroy(f);
+        return st;
+    }
+    *out = f;
+    return NB_OK;
+}
+
+int nb_force_create_bh(nb_force** out, double G, double softening, double theta, int leaf_capacity,
+                       int multipole) {
+    if (!out || !(theta >= 0.0) || leaf_capacity < 1 || leaf_capacity > 64 ||
+        (multipole != NB_MONOPOLE && multipole != NB_QUADRUPOLE) || !std::isfinite(G) ||
+        !std::isfinite(softening)) {
+        set_error("nb_force_create_bh: invalid arguments (theta=%g leaf_capacity=%d multipole=%d)",
+                  theta, leaf_capacity, multipole);
+        return NB_ERR_INVALID;
+    }
+    nb_force* f = new nb_force();
+    f->kind = 1;
+    f->G = G;
+    f->softening = softening;
+    f->soft2 = softening * softening;
+    f->theta = theta;
+    f->leaf_cap = leaf_capacity;
+    f->multipole = multipole;
+    f->name = multipole == NB_QUADRUPOLE ? "Barnes-Hut-quadrupole" : "Barnes-Hut-monopole";
+    int st = force_common_init(f);
+    if (st == NB_OK) st = bh_create(&f->bh);
+    if (st != NB_OK) {
+        nb_force_destroy(f);
+        return st;
+    }
+    *out = f;
+    return NB_OK;
+}
+
+int nb_force_destroy(nb_force* f) {
+    if (!f) return NB_OK;
+    if (f->stream) (void)hipStreamSynchronize(f->stream);
+    if (f->bh) bh_destroy(f->bh);
+    for (auto& e : f->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (f->stream) (void)hipStreamDestroy(f->stream);
+    delete f;
+    return NB_OK;
+}
+
+int nb_force_compute(nb_force* f, nb_particles* p) {
+    if (!f || !p) {
+        set_error("nb_force_compute: null argument");
+        return NB_ERR_INVALID;
+    }
+    return force_eval(f, p->n, p->m, p->x, p->y, p->z, p->ax, p->ay, p->az);
+}
+
+const char* nb_force_name(const nb_force* f) { return f ? f->name.c_str() : ""; }
+
+int nb_force_set_timing(nb_force* f, int enabled) {
+    if (!f) return NB_ERR_INVALID;
+    f->timing = enabled != 0;
+    return NB_OK;
+}
+
+int nb_force_timing(const nb_force* f, double out[4]) {
+    if (!f || !out) return NB_ERR_INVALID;
+    out[0] = f->t_total;
+    out[1] = f->t_build;
+    out[2] = f->t_walk;
+    out[3] = (double)f->calls;
+    return NB_OK;
+}
+
+int nb_force_reset_timing(nb_force* f) {
+    if (!f) return NB_ERR_INVALID;
+    f->t_total = f->t_build = f->t_walk = 0;
+    f->calls = 0;
+    return NB_OK;
+}
+
+int nb_force_bh_stats(nb_force* f, nb_bh_stats* out) {
+    if (!f || !out || f->kind != 1) {
+        set_error("nb_force_bh_stats: not a Barnes-Hut calculator");
+        return NB_ERR_INVALID;
+    }
+    return bh_stats(f->bh, f->stream, out);
+}
+
+int nb_force_bh_topology(nb_force* f, int64_t n, int32_t* leaf_depth, uint64_t* leaf_path,
+                         uint64_t* keys, uint32_t* perm) {
+    if (!f || f->kind != 1) {
+        set_error("nb_force_bh_topology: not a Barnes-Hut calculator");
+        return NB_ERR_INVALID;
+    }
+    return bh_topology(f->bh, f->stream, n, leaf_depth, leaf_path, keys, perm);
+}
+
+int nb_force_bh_nodes(nb_force* f, int64_t capacity, int64_t* count, int32_t* level,
+                      uint64_t* path, int32_t* first, int32_t* last, double* mass, double* com,
+                      double* quad) {
+    if (!f || f->kind != 1 || !count) {
+        set_error("nb_force_bh_nodes: not a Barnes-Hut calculator");
+        return NB_ERR_INVALID;
+    }
+    return bh_nodes(f->bh, f->stream, capacity, count, level, path, first, last, mass, com, quad);
+}
+
+}  // extern "C"

@@ -1,0 +1,383 @@
+"""Python host binding of libnbody_amd.so (the gfx950 N-body engine's C ABI).
+
+The classes mirror the reference's C++ interface so code written against it
+reads the same:
+
+* ``Particles``                 -- ParticleArrays + allocate/free/copy
+                                   (CUDA/include/particle_data.hpp:6-73)
+* ``ForceCalculatorNaive``      -- ForceCalculatorNaive(softening)
+                                   (CPU/include/force_calculator_naive.hpp:12)
+* ``BarnesHutForceCalculator``  -- BarnesHutForceCalculator(softening, theta, leaf_capacity)
+                                   (CUDA/include/force_calculator_barnes_hut.hpp:56)
+* ``LeapfrogIntegrator``, ``RK4Integrator`` -- TimeIntegrator::step(calculator, particles, dt)
+                                   (CUDA/include/integrator.hpp:10-14)
+
+Every call goes through the C ABI declared in include/nbody_amd.h; there is no
+CPU fallback.  If the shared library is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnbody_amd.so")
+
+# The reference's G = 4*pi^2 (CPU/include/common.hpp:28), same rounding.
+G_REF = 4.0 * math.pi * math.pi
+
+NB_DIRECT_F64_EXACT, NB_DIRECT_F64, NB_DIRECT_F32 = 0, 1, 2
+NB_MONOPOLE, NB_QUADRUPOLE = 0, 1
+NB_LEAPFROG, NB_RK4 = 0, 1
+STATUS = {0: "NB_OK", 1: "NB_ERR_INVALID", 2: "NB_ERR_HIP", 3: "NB_ERR_NOMEM",
+          4: "NB_ERR_CAPACITY", 5: "NB_ERR_BUDGET", 6: "NB_ERR_STATE"}
+PRECISIONS = {"f64_exact": NB_DIRECT_F64_EXACT, "f64": NB_DIRECT_F64, "f32": NB_DIRECT_F32}
+
+
+class NBError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Particles(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in
+                                     ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m")]
+
+
+class _BHStats(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("n_leaves", C.c_int64), ("max_level", C.c_int32),
+                ("level_limit", C.c_int32), ("oversize_cells", C.c_int64),
+                ("pp_interactions", C.c_int64), ("pc_interactions", C.c_int64),
+                ("max_frontier", C.c_int32), ("frontier_overflow", C.c_int32),
+                ("root_center", C.c_double * 3), ("root_size", C.c_double),
+                ("n_groups", C.c_int64)]
+
+
+class _Diag(C.Structure):
+    _fields_ = [("ke", C.c_double), ("pe", C.c_double), ("pe_soft", C.c_double),
+                ("p", C.c_double * 3), ("l", C.c_double * 3), ("com", C.c_double * 3),
+                ("mass", C.c_double)]
+
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_Dn = C.c_void_p  # nullable double*
+_vp = C.c_void_p
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "nb_last_error": ([], C.c_char_p),
+        "nb_version": ([], C.c_char_p),
+        "nb_device_count": ([C.POINTER(C.c_int)], C.c_int),
+        "nb_set_device": ([C.c_int], C.c_int),
+        "nb_synchronize": ([], C.c_int),
+        "nb_particles_alloc": ([C.POINTER(_Particles), C.c_int64], C.c_int),
+        "nb_particles_free": ([C.POINTER(_Particles)], C.c_int),
+        "nb_particles_upload": ([C.POINTER(_Particles)] + [_Dn] * 7, C.c_int),
+        "nb_particles_download": ([C.POINTER(_Particles)] + [_Dn] * 10, C.c_int),
+        "nb_force_create_direct": ([C.POINTER(_vp), C.c_int, C.c_double, C.c_double], C.c_int),
+        "nb_force_create_bh": ([C.POINTER(_vp), C.c_double, C.c_double, C.c_double, C.c_int,
+                                C.c_int], C.c_int),
+        "nb_force_destroy": ([_vp], C.c_int),
+        "nb_force_compute": ([_vp, C.POINTER(_Particles)], C.c_int),
+        "nb_force_name": ([_vp], C.c_char_p),
+        "nb_force_set_timing": ([_vp, C.c_int], C.c_int),
+        "nb_force_timing": ([_vp, _D], C.c_int),
+        "nb_force_reset_timing": ([_vp], C.c_int),
+        "nb_force_bh_stats": ([_vp, C.POINTER(_BHStats)], C.c_int),
+        "nb_force_bh_topology": ([_vp, C.c_int64, _vp, _vp, _vp, _vp], C.c_int),
+        "nb_force_bh_nodes": ([_vp, C.c_int64, C.POINTER(C.c_int64)] + [_vp] * 7, C.c_int),
+        "nb_integrator_create": ([C.POINTER(_vp), C.c_int], C.c_int),
+        "nb_integrator_destroy": ([_vp], C.c_int),
+        "nb_integrator_step": ([_vp, _vp, C.POINTER(_Particles), C.c_double], C.c_int),
+        "nb_integrator_run": ([_vp, _vp, C.POINTER(_Particles), C.c_double, C.c_int], C.c_int),
+        "nb_integrator_reset": ([_vp], C.c_int),
+        "nb_integrator_name": ([_vp], C.c_char_p),
+        "nb_diagnostics": ([C.POINTER(_Particles), C.c_double, C.c_double, C.c_int,
+                            C.POINTER(_Diag)], C.c_int),
+        "nb_philox4x32_10": ([_vp, _vp, _vp], None),
+        "nb_ic_plummer": ([C.c_int64, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                           C.c_double] + [_D] * 7, C.c_int),
+        "nb_ic_uniform_gaussian": ([C.c_int64, C.c_uint64, C.c_double] + [_D] * 7, C.c_int),
+        "nb_ic_plummer_disk": ([C.c_int64, C.c_uint64] + [C.c_double] * 6 + [_D] * 7, C.c_int),
+        "nb_file_read": ([C.c_char_p, C.POINTER(C.c_int64)] + [_Dn] * 7 + [C.c_int64], C.c_int),
+        "nb_file_write": ([C.c_char_p, C.c_int64] + [_D] * 7, C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    return L
+
+
+lib = _load()
+
+
+def _check(rc):
+    if rc != 0:
+        raise NBError(rc, lib.nb_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    _check(lib.nb_device_count(C.byref(c)))
+    return c.value
+
+
+def synchronize():
+    _check(lib.nb_synchronize())
+
+
+# --------------------------------------------------------------------------- state
+class Particles:
+    """Device SoA particle state (ParticleArrays)."""
+
+    def __init__(self, n: int):
+        self._p = _Particles()
+        _check(lib.nb_particles_alloc(C.byref(self._p), int(n)))
+        self.n = int(n)
+
+    @classmethod
+    def from_arrays(cls, m, x, y, z, vx=None, vy=None, vz=None):
+        p = cls(len(m))
+        p.upload(m, x, y, z, vx, vy, vz)
+        return p
+
+    def upload(self, m, x, y, z, vx=None, vy=None, vz=None):
+        arrs = [_f64(a) for a in (x, y, z)]
+        vel = [None if a is None else _f64(a) for a in (vx, vy, vz)]
+        mm = _f64(m)
+        for a in arrs + [mm] + [v for v in vel if v is not None]:
+            if len(a) != self.n:
+                raise NBError(1, f"array length {len(a)} != n={self.n}")
+        _check(lib.nb_particles_upload(C.byref(self._p), *[_ptr(a) for a in arrs],
+                                       *[_ptr(v) for v in vel], _ptr(mm)))
+
+    def download(self, fields=("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m")):
+        names = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m")
+        out = {k: np.empty(self.n) for k in fields}
+        _check(lib.nb_particles_download(C.byref(self._p),
+                                         *[_ptr(out.get(k)) for k in names]))
+        return out
+
+    def acc(self):
+        d = self.download(("ax", "ay", "az"))
+        return d["ax"], d["ay"], d["az"]
+
+    def free(self):
+        if getattr(self, "_p", None) is not None and self._p.n:
+            _check(lib.nb_particles_free(C.byref(self._p)))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def ref(self):
+        return C.byref(self._p)
+
+
+# --------------------------------------------------------------------------- forces
+class ForceCalculator:
+    _h = None
+
+    def computeAccelerations(self, particles: Particles):
+        _check(lib.nb_force_compute(self._h, particles.ref))
+
+    compute = computeAccelerations
+
+    def name(self) -> str:
+        return lib.nb_force_name(self._h).decode()
+
+    def set_timing(self, on=True):
+        _check(lib.nb_force_set_timing(self._h, int(on)))
+
+    def timing(self):
+        out = np.zeros(4)
+        _check(lib.nb_force_timing(self._h, out))
+        return dict(total_ms=out[0], build_ms=out[1], walk_ms=out[2], calls=int(out[3]))
+
+    def reset_timing(self):
+        _check(lib.nb_force_reset_timing(self._h))
+
+    def close(self):
+        if self._h:
+            lib.nb_force_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ForceCalculatorNaive(ForceCalculator):
+    def __init__(self, softening: float = 0.0, precision: str = "f64_exact", G: float = G_REF):
+        h = C.c_void_p()
+        _check(lib.nb_force_create_direct(C.byref(h), PRECISIONS[precision], float(G),
+                                          float(softening)))
+        self._h = h
+        self.softening, self.G, self.precision = softening, G, precision
+
+
+class BarnesHutForceCalculator(ForceCalculator):
+    def __init__(self, softening: float = 0.0, theta: float = 0.5, leaf_capacity: int = 8,
+                 multipole: str = "quadrupole", G: float = G_REF):
+        h = C.c_void_p()
+        mp = {"monopole": NB_MONOPOLE, "quadrupole": NB_QUADRUPOLE}[multipole]
+        _check(lib.nb_force_create_bh(C.byref(h), float(G), float(softening), float(theta),
+                                      int(leaf_capacity), mp))
+        self._h = h
+        self.softening, self.theta, self.leaf_capacity, self.G = softening, theta, leaf_capacity, G
+
+    def stats(self, check=True):
+        s = _BHStats()
+        rc = lib.nb_force_bh_stats(self._h, C.byref(s))
+        if check:
+            _check(rc)
+        return dict(n_nodes=s.n_nodes, n_leaves=s.n_leaves, max_level=s.max_level,
+                    level_limit=s.level_limit, oversize_cells=s.oversize_cells,
+                    pp_interactions=s.pp_interactions, pc_interactions=s.pc_interactions,
+                    max_frontier=s.max_frontier, frontier_overflow=s.frontier_overflow,
+                    root_center=np.array(s.root_center[:]), root_size=s.root_size,
+                    n_groups=s.n_groups, status=rc)
+
+    def topology(self, n):
+        depth = np.zeros(n, np.int32)
+        path = np.zeros(n, np.uint64)
+        keys = np.zeros(n, np.uint64)
+        perm = np.zeros(n, np.uint32)
+        _check(lib.nb_force_bh_topology(self._h, n, depth.ctypes.data, path.ctypes.data,
+                                        keys.ctypes.data, perm.ctypes.data))
+        return dict(depth=depth, path=path, keys=keys, perm=perm)
+
+    def nodes(self):
+        cnt = C.c_int64(0)
+        _check(lib.nb_force_bh_nodes(self._h, 0, C.byref(cnt), *([None] * 7)))
+        k = cnt.value
+        level = np.zeros(k, np.int32)
+        path = np.zeros(k, np.uint64)
+        first = np.zeros(k, np.int32)
+        last = np.zeros(k, np.int32)
+        mass = np.zeros(k)
+        com = np.zeros((k, 3))
+        quad = np.zeros((k, 6))
+        _check(lib.nb_force_bh_nodes(self._h, k, C.byref(cnt), level.ctypes.data,
+                                     path.ctypes.data, first.ctypes.data, last.ctypes.data,
+                                     mass.ctypes.data, com.ctypes.data, quad.ctypes.data))
+        return dict(level=level, path=path, first=first, last=last, mass=mass, com=com, quad=quad)
+
+
+# --------------------------------------------------------------------------- integrators
+class TimeIntegrator:
+    _kind = NB_LEAPFROG
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib.nb_integrator_create(C.byref(h), self._kind))
+        self._h = h
+
+    def step(self, calculator: ForceCalculator, particles: Particles, dt: float):
+        _check(lib.nb_integrator_step(self._h, calculator._h, particles.ref, float(dt)))
+
+    def run(self, calculator: ForceCalculator, particles: Particles, dt: float, steps: int):
+        _check(lib.nb_integrator_run(self._h, calculator._h, particles.ref, float(dt),
+                                     int(steps)))
+
+    def reset(self):
+        _check(lib.nb_integrator_reset(self._h))
+
+    def name(self) -> str:
+        return lib.nb_integrator_name(self._h).decode()
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib.nb_integrator_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class LeapfrogIntegrator(TimeIntegrator):
+    _kind = NB_LEAPFROG
+
+
+class RK4Integrator(TimeIntegrator):
+    _kind = NB_RK4
+
+
+# --------------------------------------------------------------------------- diagnostics
+def diagnostics(particles: Particles, G: float = G_REF, softening: float = 0.0,
+                with_potential: bool = True) -> dict:
+    d = _Diag()
+    _check(lib.nb_diagnostics(particles.ref, float(G), float(softening), int(with_potential),
+                              C.byref(d)))
+    return dict(ke=d.ke, pe=d.pe, pe_soft=d.pe_soft, p=np.array(d.p[:]), l=np.array(d.l[:]),
+                com=np.array(d.com[:]), mass=d.mass)
+
+
+# --------------------------------------------------------------------------- initial conditions
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib.nb_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def _alloc7(n):
+    return [np.zeros(n) for _ in range(7)]
+
+
+def ic_plummer(n, seed, a=1.0, M=1.0, G=1.0, rmax=0.0):
+    arr = _alloc7(n)
+    _check(lib.nb_ic_plummer(int(n), int(seed), a, M, G, rmax, *arr))
+    return dict(zip(("m", "x", "y", "z", "vx", "vy", "vz"), arr))
+
+
+def ic_uniform_gaussian(n, seed, sigma=0.1):
+    arr = _alloc7(n)
+    _check(lib.nb_ic_uniform_gaussian(int(n), int(seed), sigma, *arr))
+    return dict(zip(("m", "x", "y", "z", "vx", "vy", "vz"), arr))
+
+
+def ic_plummer_disk(n, seed, a=1.0, M=1.0, G=1.0, disk_frac=0.1, rd=1.0, zd=0.05):
+    arr = _alloc7(n)
+    _check(lib.nb_ic_plummer_disk(int(n), int(seed), a, M, G, disk_frac, rd, zd, *arr))
+    return dict(zip(("m", "x", "y", "z", "vx", "vy", "vz"), arr))
+
+
+# --------------------------------------------------------------------------- particle files
+def read_particles(path) -> dict:
+    n = C.c_int64(0)
+    _check(lib.nb_file_read(os.fsencode(path), C.byref(n), *([None] * 7), 0))
+    arr = _alloc7(n.value)
+    _check(lib.nb_file_read(os.fsencode(path), C.byref(n), *[a.ctypes.data for a in arr],
+                            n.value))
+    return dict(zip(("m", "x", "y", "z", "vx", "vy", "vz"), arr))
+
+
+def write_particles(path, m, x, y, z, vx=None, vy=None, vz=None):
+    n = len(m)
+    zero = np.zeros(n)
+    arrs = [_f64(a) for a in (m, x, y, z)] + [zero if v is None else _f64(v) for v in (vx, vy, vz)]
+    _check(lib.nb_file_write(os.fsencode(path), n, *arrs))
