@@ -116,13 +116,18 @@ int nb_force_compute(nb_force* f, nb_particles* p);
 /* ForceCalculator::name() (CPU/include/force_calculator.hpp:28). */
 const char* nb_force_name(const nb_force* f);
 
-/* Timing of the calls so far, in ms measured with HIP events
- * (getTimeStatistics, CPU/include/force_calculator.hpp:30; CUDA
- * BarnesHutForceCalculator getters, CUDA/include/force_calculator_barnes_hut.hpp:120-125).
- * out[0]=total, out[1]=tree build (bbox..up-sweep), out[2]=walk, out[3]=calls.
- * Timing events are recorded only when enabled (they add stream syncs). */
+/* Timing of the calls made while timing was enabled, in ms measured with HIP
+ * events on the calculator's stream (getTimeStatistics,
+ * CPU/include/force_calculator.hpp:30; CUDA BarnesHutForceCalculator getters,
+ * CUDA/include/force_calculator_barnes_hut.hpp:120-125).  Recording adds no
+ * synchronisation; reading the timings synchronizes the stream.
+ * nb_force_timing: out[0]=total, out[1]=tree build (bbox..up-sweep),
+ * out[2]=walk (the whole direct sum for direct calculators), out[3]=calls.
+ * nb_force_stage_timing: out[0..6] = bbox, keys, sort, gather, build,
+ * up-sweep, walk; out[7] = calls (if n_out > 7). */
 int nb_force_set_timing(nb_force* f, int enabled);
-int nb_force_timing(const nb_force* f, double out[4]);
+int nb_force_timing(nb_force* f, double out[4]);
+int nb_force_stage_timing(nb_force* f, double* out, int n_out);
 int nb_force_reset_timing(nb_force* f);
 
 /* Statistics of the last Barnes-Hut evaluation (synchronizes). */

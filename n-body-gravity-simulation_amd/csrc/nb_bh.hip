@@ -87,7 +87,7 @@ struct nb::BHWorkspace {
     DevBuf root, bbox_part, keys0, keys1, vals0, vals1, hist, spos;
     DevBuf lvl_leaf, lvl_sh, blkcnt, blkoff, levels, leaf_of, counters;
     DevBuf n_pstart, n_pend, n_child, n_cend, n_level, n_mom, w_mac, w_link, w_q0, w_q1;
-    DevBuf walk_scratch, gflag, gstart;
+    DevBuf walk_scratch, gflag, gstart, glev;
     int walk_blocks = 0;
 };
 
@@ -381,7 +381,8 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
                                                  int cap, const RootInfo* __restrict__ root,
                                                  signed char* __restrict__ Lout,
                                                  signed char* __restrict__ shout,
-                                                 unsigned char* __restrict__ gflag, int nblk,
+                                                 unsigned char* __restrict__ gflag,
+                                                 signed char* __restrict__ glev, int nblk,
                                                  int* __restrict__ blkcnt,
                                                  Counters* __restrict__ ctr) {
     constexpr int H = kGroup;  // key halo on both sides of the block
@@ -424,6 +425,7 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
         shi = (i == 0) ? -1 : shared_levels(sk[H + t - 1], sk[H + t]);
         Lout[i] = (signed char)Li;
         shout[i] = (signed char)shi;
+        glev[i] = (signed char)gl;
     }
     const bool leaf_start = valid && shi < Li;
     const bool group_start = valid && shi < gl;
@@ -547,6 +549,7 @@ __global__ __launch_bounds__(kBlk) void k_emit(
         }
     }
 }
+
 
 // ------------------------------------------------------------------ up-sweep
 __global__ __launch_bounds__(kBlk) void k_upsweep(
@@ -715,10 +718,17 @@ __global__ __launch_bounds__(64) void k_walk(int n, const float4* __restrict__ s
         if (grp >= ngroups) break;
         const int gend = gstart[grp + 1];
       for (int g0 = gstart[grp]; g0 < gend; g0 += 64) {  // groups above 64 (oversize leaves) in chunks
-        const int k = g0 + lane;
-        const bool active = k < gend;
         const int nact = min(64, gend - g0);
-        const float4 me = spos[active ? k : g0];
+        // A group of nact <= 32 targets runs on nsub = 64/nact lane subsets; each
+        // subset takes every nsub-th entry of the interaction lists and the
+        // partial sums are added in subset order at the end.
+        const int nsub = 64 / nact;
+        const int sub = lane / nact;
+        const int tix = lane - sub * nact;
+        const bool live = sub < nsub;
+        const int k = g0 + tix;
+        const bool active = live && sub == 0;
+        const float4 me = spos[live ? k : g0];
 
         // Group bounding box.
         float lo0 = me.x, lo1 = me.y, lo2 = me.z, hi0 = me.x, hi1 = me.y, hi2 = me.z;
@@ -741,7 +751,7 @@ __global__ __launch_bounds__(64) void k_walk(int n, const float4* __restrict__ s
         int2* nxt = fr0;
 
         auto flush_particles = [&](int cnt) {
-            for (int j = 0; j < cnt; ++j) {
+            for (int j = live ? sub : cnt; j < cnt; j += nsub) {
                 const float4 s = pl[j];
                 const float dx = s.x - me.x, dy = s.y - me.y, dz = s.z - me.z;
                 const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
@@ -763,7 +773,7 @@ __global__ __launch_bounds__(64) void k_walk(int n, const float4* __restrict__ s
                 if (kQuad) st2[lane] = q1[id];
             }
             __syncthreads();
-            for (int j = 0; j < cnt; ++j) {
+            for (int j = live ? sub : cnt; j < cnt; j += nsub) {
                 const float4 c = st0[j];
                 const float4 qa = st1[j];
                 const float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
@@ -896,6 +906,19 @@ __global__ __launch_bounds__(64) void k_walk(int n, const float4* __restrict__ s
         if (ncl > 0) flush_cells(ncl);
         if (npl > 0) flush_particles(npl);
 
+        if (nsub > 1) {  // reduce the subsets' partial sums in subset order
+            st0[lane] = make_float4(fx, fy, fz, 0.f);
+            __syncthreads();
+            if (active) {
+                for (int q = 1; q < nsub; ++q) {
+                    const float4 v = st0[q * nact + tix];
+                    fx += v.x;
+                    fy += v.y;
+                    fz += v.z;
+                }
+            }
+            __syncthreads();
+        }
         if (active) {
             const unsigned j = perm[k];
             ax[j] = G * (double)fx;
@@ -928,7 +951,7 @@ static int64_t node_capacity(int64_t n, int cap) {
 
 int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, int multipole,
             int64_t n64, const double* m, const double* x, const double* y, const double* z,
-            double* ax, double* ay, double* az, hipStream_t s, hipEvent_t ev_build_done) {
+            double* ax, double* ay, double* az, hipStream_t s, const hipEvent_t* ev) {
     if (n64 <= 0) return NB_OK;
     if (n64 > (int64_t)1 << 28) {
         set_error("Barnes-Hut: n=%lld exceeds 2^28", (long long)n64);
@@ -959,6 +982,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_TRY(w->blkcnt.reserve((size_t)nblk * kCols * 4));
     NB_TRY(w->gflag.reserve((size_t)n));
     NB_TRY(w->gstart.reserve((size_t)(n + 1) * 4));
+    NB_TRY(w->glev.reserve((size_t)n));
     NB_TRY(w->leaf_of.reserve((size_t)n * 4));
     NB_TRY(w->n_pstart.reserve((size_t)ncap * 4));
     NB_TRY(w->n_pend.reserve((size_t)ncap * 4));
@@ -984,8 +1008,10 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->bbox_part.as<double>());
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlk), 0, s, nbb, w->bbox_part.as<double>(),
                        root);
+    if (ev) NB_HIP(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(k_keys, dim3(nblk), dim3(kBlk), 0, s, n, x, y, z, root, k0, v0);
     NB_LAUNCH_CHECK();
+    if (ev) NB_HIP(hipEventRecord(ev[2], s));
     for (int p = 0; p < kSortPasses; ++p) {
         const int shift = 8 * p;
         hipLaunchKernelGGL(k_radix_hist, dim3(nsort), dim3(kBlk), 0, s, n, k0, shift, nsort,
@@ -999,11 +1025,14 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     }
     NB_LAUNCH_CHECK();
     // kSortPasses is even: the sorted keys are back in keys0/vals0.
+    if (ev) NB_HIP(hipEventRecord(ev[3], s));
     hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlk), 0, s, n, v0, x, y, z, m,
                        w->spos.as<float4>());
+    if (ev) NB_HIP(hipEventRecord(ev[4], s));
     hipLaunchKernelGGL(k_levels, dim3(nblk), dim3(kBlk), 0, s, n, k0, leaf_cap, root,
                        w->lvl_leaf.as<signed char>(), w->lvl_sh.as<signed char>(),
-                       w->gflag.as<unsigned char>(), nblk, w->blkcnt.as<int>(), ctr);
+                       w->gflag.as<unsigned char>(), w->glev.as<signed char>(), nblk,
+                       w->blkcnt.as<int>(), ctr);
     hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(1024), 0, s, n, nblk, w->blkcnt.as<int>(), li,
                        (int)ncap, w->gstart.as<int>(), ctr);
     hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(kBlk), 0, s, n, w->lvl_leaf.as<signed char>(),
@@ -1012,6 +1041,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->n_cend.as<int>(), w->n_level.as<signed char>(), w->leaf_of.as<int>(),
                        w->gflag.as<unsigned char>(), w->gstart.as<int>(), ctr);
     NB_LAUNCH_CHECK();
+    if (ev) NB_HIP(hipEventRecord(ev[5], s));
     const double inv_theta2 = theta > 0.0 ? 1.0 / (theta * theta) : INFINITY;
     for (int l = kLevels - 1; l >= 0; --l) {
         hipLaunchKernelGGL(k_upsweep, dim3(kUpsweepGrid), dim3(kBlk), 0, s, l, li, root,
@@ -1022,7 +1052,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                            n, ctr);
     }
     NB_LAUNCH_CHECK();
-    if (ev_build_done) NB_HIP(hipEventRecord(ev_build_done, s));
+    if (ev) NB_HIP(hipEventRecord(ev[6], s));
 
     const int ngroups = grid_for(n, 16);  // upper bound on walk groups in flight
     if (w->walk_blocks == 0) {
@@ -1049,6 +1079,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     }
 #undef NB_WALK
     NB_LAUNCH_CHECK();
+    if (ev) NB_HIP(hipEventRecord(ev[7], s));
     w->last_n = n;
     w->last_cap = leaf_cap;
     w->last_valid = 1;

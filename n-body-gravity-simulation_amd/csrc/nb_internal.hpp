@@ -73,7 +73,7 @@ int bh_create(BHWorkspace** w);
 void bh_destroy(BHWorkspace* w);
 int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, int multipole,
             int64_t n, const double* m, const double* x, const double* y, const double* z,
-            double* ax, double* ay, double* az, hipStream_t s, hipEvent_t ev_build_done);
+            double* ax, double* ay, double* az, hipStream_t s, const hipEvent_t* stage_ev);
 int bh_stats(BHWorkspace* w, hipStream_t s, nb_bh_stats* out);
 int bh_topology(BHWorkspace* w, hipStream_t s, int64_t n, int32_t* depth, uint64_t* path,
                 uint64_t* keys, uint32_t* perm);
@@ -95,14 +95,24 @@ struct nb_force {
     nb::DirectWS direct;
     nb::BHWorkspace* bh = nullptr;
     std::string name;
-    // timing (HIP events; only recorded when enabled)
+    // Timing: when enabled, every call records kStages+1 HIP events taken from
+    // a pool on the calculator's stream (no synchronisation); the elapsed times
+    // are resolved when the timings are read.
     bool timing = false;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    double t_total = 0, t_build = 0, t_walk = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_pending;  // first pool index of each unresolved call
+    size_t ev_used = 0;
+    double t_stage[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t timed_calls = 0;
     int64_t calls = 0;
 };
 
 namespace nb {
+// Barnes-Hut stages timed separately (a direct sum is one "walk" stage):
+// bbox, keys, sort, gather, build, up-sweep, walk.
+constexpr int kStages = 7;
+// Resolves pending timing events into f->t_stage (synchronizes f->stream).
+int resolve_timing(nb_force* f);
 // Force evaluation on arbitrary position arrays (used by the integrators for
 // intermediate stages).
 int force_eval(nb_force* f, int64_t n, const double* m, const double* x, const double* y,

@@ -40,34 +40,47 @@ void DevBuf::release() {
     bytes = 0;
 }
 
+int resolve_timing(nb_force* f) {
+    if (f->ev_pending.empty()) return NB_OK;
+    NB_HIP(hipStreamSynchronize(f->stream));
+    for (int b : f->ev_pending) {
+        for (int k = 0; k < kStages; ++k) {
+            float ms = 0.f;
+            NB_HIP(hipEventElapsedTime(&ms, f->ev_pool[b + k], f->ev_pool[b + k + 1]));
+            f->t_stage[k] += ms;
+        }
+    }
+    f->timed_calls += (int64_t)f->ev_pending.size();
+    f->ev_pending.clear();
+    f->ev_used = 0;
+    return NB_OK;
+}
+
 int force_eval(nb_force* f, int64_t n, const double* m, const double* x, const double* y,
                const double* z, double* ax, double* ay, double* az) {
     if (n <= 0) return NB_OK;
-    const bool timed = f->timing;
-    if (timed) NB_HIP(hipEventRecord(f->ev[0], f->stream));
-    int st;
-    if (f->kind == 0) {
-        st = direct_eval(f->precision, f->G, f->soft2, n, m, x, y, z, ax, ay, az, f->direct,
-                         f->stream);
-        if (st == NB_OK && timed) NB_HIP(hipEventRecord(f->ev[1], f->stream));
-    } else {
-        st = bh_eval(f->bh, f->G, f->soft2, f->theta, f->leaf_cap, f->multipole, n, m, x, y, z,
-                     ax, ay, az, f->stream, timed ? f->ev[1] : nullptr);
-    }
-    if (st != NB_OK) return st;
-    if (timed) {
-        NB_HIP(hipEventRecord(f->ev[2], f->stream));
-        NB_HIP(hipEventSynchronize(f->ev[2]));
-        float a = 0, b = 0;
-        NB_HIP(hipEventElapsedTime(&a, f->ev[0], f->ev[1]));
-        NB_HIP(hipEventElapsedTime(&b, f->ev[1], f->ev[2]));
-        if (f->kind == 0) {
-            f->t_walk += a;  // the whole direct sum is the "walk" phase
-        } else {
-            f->t_build += a;
-            f->t_walk += b;
+    const hipEvent_t* ev = nullptr;
+    if (f->timing) {
+        if (f->ev_pending.size() >= 256) NB_TRY(resolve_timing(f));  // bound the event pool
+        const size_t need = f->ev_used + kStages + 1;
+        while (f->ev_pool.size() < need) {
+            hipEvent_t e;
+            NB_HIP(hipEventCreate(&e));
+            f->ev_pool.push_back(e);
         }
-        f->t_total += a + b;
+        f->ev_pending.push_back((int)f->ev_used);
+        ev = &f->ev_pool[f->ev_used];
+        f->ev_used = need;
+        NB_HIP(hipEventRecord(ev[0], f->stream));
+    }
+    if (f->kind == 0) {
+        NB_TRY(direct_eval(f->precision, f->G, f->soft2, n, m, x, y, z, ax, ay, az, f->direct,
+                           f->stream));
+        if (ev)  // a direct sum has no tree stages: all time is booked as "walk"
+            for (int k = 1; k <= kStages; ++k) NB_HIP(hipEventRecord(ev[k], f->stream));
+    } else {
+        NB_TRY(bh_eval(f->bh, f->G, f->soft2, f->theta, f->leaf_cap, f->multipole, n, m, x, y, z,
+                       ax, ay, az, f->stream, ev));
     }
     f->calls++;
     return NB_OK;
@@ -179,7 +192,6 @@ int nb_particles_download(const nb_particles* p, double* x, double* y, double* z
 static int force_common_init(nb_force* f) {
     NB_HIP(hipGetDevice(&f->device));
     NB_HIP(hipStreamCreate(&f->stream));
-    for (auto& e : f->ev) NB_HIP(hipEventCreate(&e));
     return NB_OK;
 }
 
@@ -239,8 +251,7 @@ int nb_force_destroy(nb_force* f) {
     if (!f) return NB_OK;
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     if (f->bh) bh_destroy(f->bh);
-    for (auto& e : f->ev)
-        if (e) (void)hipEventDestroy(e);
+    for (auto& e : f->ev_pool) (void)hipEventDestroy(e);
     if (f->stream) (void)hipStreamDestroy(f->stream);
     delete f;
     return NB_OK;
@@ -258,23 +269,37 @@ const char* nb_force_name(const nb_force* f) { return f ? f->name.c_str() : ""; 
 
 int nb_force_set_timing(nb_force* f, int enabled) {
     if (!f) return NB_ERR_INVALID;
+    if (!enabled) NB_TRY(resolve_timing(f));
     f->timing = enabled != 0;
     return NB_OK;
 }
 
-int nb_force_timing(const nb_force* f, double out[4]) {
+int nb_force_timing(nb_force* f, double out[4]) {
     if (!f || !out) return NB_ERR_INVALID;
-    out[0] = f->t_total;
-    out[1] = f->t_build;
-    out[2] = f->t_walk;
-    out[3] = (double)f->calls;
+    NB_TRY(resolve_timing(f));
+    double total = 0, build = 0;
+    for (int k = 0; k < kStages; ++k) total += f->t_stage[k];
+    for (int k = 0; k < kStages - 1; ++k) build += f->t_stage[k];
+    out[0] = total;
+    out[1] = build;
+    out[2] = f->t_stage[kStages - 1];
+    out[3] = (double)f->timed_calls;
+    return NB_OK;
+}
+
+int nb_force_stage_timing(nb_force* f, double* out, int n_out) {
+    if (!f || !out || n_out < 1) return NB_ERR_INVALID;
+    NB_TRY(resolve_timing(f));
+    for (int k = 0; k < n_out && k < kStages; ++k) out[k] = f->t_stage[k];
+    if (n_out > kStages) out[kStages] = (double)f->timed_calls;
     return NB_OK;
 }
 
 int nb_force_reset_timing(nb_force* f) {
     if (!f) return NB_ERR_INVALID;
-    f->t_total = f->t_build = f->t_walk = 0;
-    f->calls = 0;
+    NB_TRY(resolve_timing(f));
+    for (double& t : f->t_stage) t = 0;
+    f->timed_calls = 0;
     return NB_OK;
 }
 

@@ -90,6 +90,7 @@ def _load():
         "nb_force_name": ([_vp], C.c_char_p),
         "nb_force_set_timing": ([_vp, C.c_int], C.c_int),
         "nb_force_timing": ([_vp, _D], C.c_int),
+        "nb_force_stage_timing": ([_vp, _D, C.c_int], C.c_int),
         "nb_force_reset_timing": ([_vp], C.c_int),
         "nb_force_bh_stats": ([_vp, C.POINTER(_BHStats)], C.c_int),
         "nb_force_bh_topology": ([_vp, C.c_int64, _vp, _vp, _vp, _vp], C.c_int),
@@ -213,6 +214,17 @@ class ForceCalculator:
         out = np.zeros(4)
         _check(lib.nb_force_timing(self._h, out))
         return dict(total_ms=out[0], build_ms=out[1], walk_ms=out[2], calls=int(out[3]))
+
+    STAGES = ("bbox", "keys", "sort", "gather", "build", "upsweep", "walk")
+
+    def stage_timing(self):
+        """Accumulated ms per stage (bbox, keys, sort, gather, build, upsweep,
+        walk) and the number of timed calls."""
+        out = np.zeros(8)
+        _check(lib.nb_force_stage_timing(self._h, out, 8))
+        d = {k: float(v) for k, v in zip(self.STAGES, out[:7])}
+        d["calls"] = int(out[7])
+        return d
 
     def reset_timing(self):
         _check(lib.nb_force_reset_timing(self._h))

@@ -1,0 +1,192 @@
+"""GPU parity of the direct-sum kernels and the integrators, through the C ABI.
+
+Bars: FP64-exact kernel and the leapfrog/RK4 updates are bit-identical to the
+oracle and to the compiled-reference goldens; the fast FP64 kernel is within
+1e-13 relative; the FP32 kernel within 1e-5 (p99) relative of the FP64 sum.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_err, sha256
+
+pytestmark = pytest.mark.gpu
+G_REF = 4.0 * math.pi * math.pi
+STATE = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az")
+
+
+def gpu_force(nb, d, precision, eps, G=G_REF):
+    f = nb.ForceCalculatorNaive(eps, precision, G=G)
+    p = nb.Particles.from_arrays(d["m"], d["x"], d["y"], d["z"])
+    f.computeAccelerations(p)
+    return p.acc()
+
+
+@pytest.mark.parametrize("fx", ["p100", "p1000", "c1"])
+@pytest.mark.parametrize("eps", [0.0, 0.01])
+def test_f64_exact_bitwise_vs_compiled_reference(nb, golden, fixtures, fx, eps):
+    a = gpu_force(nb, fixtures[fx], "f64_exact", eps)
+    assert sha256(*a) == golden[f"direct/{fx}/eps{eps}"]["sha256"]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 255, 256, 257, 1000, 2049])
+@pytest.mark.parametrize("eps", [0.0, 0.05])
+def test_f64_exact_bitwise_vs_oracle_ragged(nb, O, n, eps):
+    rng = np.random.default_rng(n)
+    m = rng.uniform(0.5, 2.0, n)
+    x, y, z = rng.normal(size=(3, n))
+    d = dict(m=m, x=x, y=y, z=z)
+    a = gpu_force(nb, d, "f64_exact", eps, G=1.0)
+    r = O.direct(m, x, y, z, 1.0, eps * eps)
+    for u, v in zip(a, r):
+        assert np.array_equal(u, v)
+
+
+def test_single_particle_has_zero_acceleration(nb):
+    for prec in ("f64_exact", "f64", "f32"):
+        a = gpu_force(nb, dict(m=np.ones(1), x=np.ones(1), y=np.ones(1), z=np.ones(1)), prec, 0.0)
+        assert all(v[0] == 0.0 for v in a)
+
+
+def test_empty_system(nb):
+    f = nb.ForceCalculatorNaive(0.0, "f32")
+    p = nb.Particles(0)
+    f.computeAccelerations(p)
+    nb.LeapfrogIntegrator().step(f, p, 1e-3)
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.01])
+def test_f64_fast_tolerance(nb, O, fixtures, eps):
+    d = fixtures["p1000"]
+    e = rel_err(gpu_force(nb, d, "f64", eps), O.direct(d["m"], d["x"], d["y"], d["z"], G_REF, eps * eps))
+    assert e.max() < 1e-13
+
+
+@pytest.mark.parametrize("fx", ["p1000", "c1"])
+@pytest.mark.parametrize("eps", [0.0, 0.01])
+def test_f32_tolerance(nb, O, fixtures, fx, eps):
+    d = fixtures[fx]
+    e = rel_err(gpu_force(nb, d, "f32", eps), O.direct(d["m"], d["x"], d["y"], d["z"], G_REF, eps * eps))
+    assert np.percentile(e, 99) < 1e-5 and e.max() < 1e-4
+
+
+def test_f32_two_clusters_and_coincident_pairs(nb, O):
+    """Separated clusters and exactly coincident particles with eps>0 (the
+    coincident pair must add exactly zero).  FP32 force arithmetic resolves
+    positions to 2^-24 |x|, so the bar holds while cluster size / |x| stays
+    above ~1e-2 (here 0.5 / 50)."""
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(3, 300)) * 0.5
+    b = rng.normal(size=(3, 300)) * 0.5 + 50.0
+    x, y, z = np.concatenate([a, b, a[:, :10]], axis=1)
+    m = np.ones(x.size)
+    d = dict(m=m, x=x, y=y, z=z)
+    r = O.direct(m, x, y, z, 1.0, 1e-4)
+    # p99 and max bars; the max is set by particles near a cluster centre whose
+    # net force nearly cancels.
+    # FP32 bar: position rounding 2^-24 * 50 over cluster size 0.5 is ~1e-5
+    # per coordinate, amplified where forces cancel.
+    for prec, p99, mx in (("f32", 2e-4, 2e-3), ("f64", 1e-13, 1e-12)):
+        e = rel_err(gpu_force(nb, d, prec, 0.01, G=1.0), r)
+        assert np.percentile(e, 99) < p99 and e.max() < mx, prec
+
+
+def test_f32_large_n_size_independent_properties(nb):
+    """N = 262144 (the Plummer + disk workload): Newton's third law makes the
+    total force vanish; compare with the fast FP64 kernel on a sample."""
+    d = nb.ic_plummer_disk(262144, 5)
+    a32 = np.stack(gpu_force(nb, d, "f32", 0.01, G=1.0), 1)
+    a64 = np.stack(gpu_force(nb, d, "f64", 0.01, G=1.0), 1)
+    e = np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1)
+    assert np.percentile(e, 99) < 1e-4
+    f = a64 * d["m"][:, None]
+    assert np.abs(f.sum(0)).max() < 1e-9 * np.abs(f).sum()
+
+
+# ------------------------------------------------------------------ integrators
+def run_gpu(nb, d, integ, prec, eps, dt, steps, G, stepwise=False):
+    f = nb.ForceCalculatorNaive(eps, prec, G=G)
+    p = nb.Particles.from_arrays(d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"])
+    it = nb.LeapfrogIntegrator() if integ == "leapfrog" else nb.RK4Integrator()
+    if stepwise:
+        for _ in range(steps):
+            it.step(f, p, dt)
+    else:
+        it.run(f, p, dt, steps)
+    g = p.download()
+    return [g[k] for k in STATE]
+
+
+@pytest.mark.parametrize("fx,steps", [("p1000", 10), ("c1", 100)])
+def test_leapfrog_bitwise_vs_compiled_reference(nb, golden, fixtures, fx, steps):
+    st = run_gpu(nb, fixtures[fx], "leapfrog", "f64_exact", 0.01, 1e-3, steps, G_REF)
+    assert sha256(*st) == golden[f"leapfrog/{fx}/eps0.01/dt1e-3/steps{steps}"]["sha256"]
+
+
+@pytest.mark.parametrize("fx,steps", [("p1000", 5), ("c1", 10)])
+def test_rk4_bitwise_vs_compiled_reference(nb, golden, fixtures, fx, steps):
+    st = run_gpu(nb, fixtures[fx], "rk4", "f64_exact", 0.01, 1e-3, steps, G_REF)
+    assert sha256(*st) == golden[f"rk4/{fx}/eps0.01/dt1e-3/steps{steps}"]["sha256"]
+
+
+def test_config1_leapfrog_g1_bitwise_vs_oracle(nb, O, fixtures):
+    """The plain direct FP64 workload: N=1000, m=1/N, G=1, eps=0.01, leapfrog
+    dt=1e-3 for 100 steps -- every bit of the final state."""
+    d = fixtures["c1"]
+    st = run_gpu(nb, d, "leapfrog", "f64_exact", 0.01, 1e-3, 100, 1.0)
+    ref = O.leapfrog_direct(d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], 1e-3, 100,
+                            1.0, 1e-4, True)
+    for k, a, b in zip(STATE, st, ref):
+        assert np.array_equal(a, b), k
+
+
+@pytest.mark.parametrize("integ", ["leapfrog", "rk4"])
+def test_step_by_step_equals_fused_run(nb, fixtures, integ):
+    d = fixtures["c1"]
+    a = run_gpu(nb, d, integ, "f32", 0.01, 1e-3, 7, 1.0, stepwise=True)
+    b = run_gpu(nb, d, integ, "f32", 0.01, 1e-3, 7, 1.0, stepwise=False)
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+
+
+def test_rk4_f32_force_close_to_f64(nb, fixtures):
+    d = fixtures["c1"]
+    a = run_gpu(nb, d, "rk4", "f32", 0.01, 1e-3, 20, 1.0)
+    b = run_gpu(nb, d, "rk4", "f64_exact", 0.01, 1e-3, 20, 1.0)
+    for u, v in zip(a[:3], b[:3]):
+        assert np.abs(u - v).max() < 1e-9
+
+
+def test_energy_conservation(nb):
+    """Leapfrog and RK4 on a Plummer sphere: relative energy drift stays small
+    (FP64 state, FP32 forces)."""
+    d = nb.ic_plummer(4096, 21)
+    for integ, tol in (("leapfrog", 1e-4), ("rk4", 1e-5)):
+        f = nb.ForceCalculatorNaive(0.01, "f32", G=1.0)
+        p = nb.Particles.from_arrays(d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"])
+        e0 = nb.diagnostics(p, 1.0, 0.01)
+        it = nb.LeapfrogIntegrator() if integ == "leapfrog" else nb.RK4Integrator()
+        it.run(f, p, 1e-3, 100)
+        e1 = nb.diagnostics(p, 1.0, 0.01)
+        E0 = e0["ke"] + e0["pe_soft"]
+        E1 = e1["ke"] + e1["pe_soft"]
+        assert abs(E1 - E0) / abs(E0) < tol, integ
+        assert np.abs(e1["p"] - e0["p"]).max() < 1e-6
+
+
+def test_diagnostics_vs_oracle(nb, O, fixtures):
+    d = fixtures["c1"]
+    p = nb.Particles.from_arrays(d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"])
+    g = nb.diagnostics(p, 1.0, 0.01)
+    o = O.diag(d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], 1.0, 1e-4)
+    for k in ("ke", "pe", "pe_soft", "mass"):
+        assert g[k] == pytest.approx(o[k], rel=1e-14), k
+    for k in ("p", "l", "com"):
+        assert np.allclose(g[k], o[k], rtol=1e-12, atol=1e-16), k
+
+
+def test_invalid_state_is_reported(nb):
+    bh = nb.BarnesHutForceCalculator(0.01, 0.5, 8)
+    with pytest.raises(nb.NBError):
+        bh.stats()  # no evaluation yet
