@@ -9,7 +9,7 @@
 //             Sorting the keys orders particles by the CPU tree's child order.
 //   sort      LSD radix sort of (key, index), 8 passes of 8 bits, stable.
 //   gather    sorted float4 (x, y, z, m).
-//   build     k_levels + k_level_scan + k_emit: the tree the reference's
+//   build     k_levels + k_col_scan + k_level_info + k_emit: the tree the reference's
 //             insertion build produces (a cell splits iff it holds more than
 //             leaf_capacity particles and its half size is >= 1e-9), built from
 //             the sorted keys without host synchronisation.  A cell at level l
@@ -34,6 +34,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 
 #include "nb_internal.hpp"
 
@@ -46,11 +47,13 @@ constexpr int kSortRows = 16;
 constexpr int kSortTile = kBlk * kSortRows;
 constexpr int kSortBins = 256;
 constexpr int kSortPasses = 8;
-constexpr int kWalkFcap = 8192;  // walk frontier capacity per block (opened cells per level)
-constexpr int kWalkBlocksPerCU = 16;
+constexpr int kWalkFcap = 4096;  // walk frontier capacity per block (opened cells per level)
+constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
 constexpr int kUpsweepGrid = 512;
 constexpr int kGroup = 64;      // walk group: cells of at most 64 particles
-constexpr int kCols = kLevels + 1;  // per-block count columns: 22 levels + group starts
+// Per-block count columns: 22 levels, group starts, leaf starts, oversize leaves.
+constexpr int kColGroup = kLevels, kColLeaf = kLevels + 1, kColOver = kLevels + 2;
+constexpr int kCols = kLevels + 3;
 
 struct RootInfo {
     double center[3];
@@ -87,7 +90,7 @@ struct nb::BHWorkspace {
     DevBuf root, bbox_part, keys0, keys1, vals0, vals1, hist, spos;
     DevBuf lvl_leaf, lvl_sh, blkcnt, blkoff, levels, leaf_of, counters;
     DevBuf n_pstart, n_pend, n_child, n_cend, n_level, n_mom, w_mac, w_link, w_q0, w_q1;
-    DevBuf walk_scratch, gflag, gstart, glev;
+    DevBuf walk_scratch, gflag, gstart, glev, totals;
     int walk_blocks = 0;
 };
 
@@ -115,39 +118,6 @@ __device__ __forceinline__ int shared_levels(unsigned long long a, unsigned long
     const unsigned long long d = a ^ b;
     if (d == 0) return kMaxDigits;
     return (__clzll((long long)d) - 1) / 3;
-}
-
-// Exclusive scan of `count` ints with one 1024-thread block.  Returns total.
-__device__ int block_scan_1024(const int* in, int* out, int count, int* sh_wave /*[16]*/) {
-    const int tid = threadIdx.x;
-    const int per = (count + 1023) / 1024;
-    const int b = tid * per;
-    const int e = min(b + per, count);
-    int local = 0;
-    for (int k = b; k < e; ++k) local += in[k];
-    const int incl = wave_incl_scan(local);
-    const int w = tid >> 6;
-    if (lane_id() == 63) sh_wave[w] = incl;
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int k = 0; k < 16; ++k) {
-            const int t = sh_wave[k];
-            sh_wave[k] = run;
-            run += t;
-        }
-        sh_wave[16] = run;
-    }
-    __syncthreads();
-    int run = sh_wave[w] + incl - local;
-    for (int k = b; k < e; ++k) {
-        const int t = in[k];
-        out[k] = run;
-        run += t;
-    }
-    const int total = sh_wave[16];
-    __syncthreads();
-    return total;
 }
 
 // ------------------------------------------------------------------ counters
@@ -306,9 +276,33 @@ __global__ __launch_bounds__(kBlk) void k_radix_hist(int n, const unsigned long 
     hist[t * nblk + blockIdx.x] = h[t];
 }
 
-__global__ __launch_bounds__(1024) void k_radix_scan(int count, int* __restrict__ hist) {
-    __shared__ int shw[17];
-    block_scan_1024(hist, hist, count, shw);
+// Exclusive scan of every column of a column-major count table
+// cnt[c * nrows + r] (one 1024-thread block per column, coalesced 1024-row
+// tiles with a running carry); the column sums go to totals[c].
+__global__ __launch_bounds__(1024) void k_col_scan(int nrows, int* __restrict__ cnt,
+                                                   int* __restrict__ totals) {
+    __shared__ int shw[16];
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    int* col = cnt + (size_t)blockIdx.x * nrows;
+    int carry = 0;
+    for (int r0 = 0; r0 < nrows; r0 += 1024) {
+        const int r = r0 + t;
+        const int v = r < nrows ? col[r] : 0;
+        const int incl = wave_incl_scan(v);
+        if (lane_id() == 63) shw[w] = incl;
+        __syncthreads();
+        int before = 0, tile = 0;
+        for (int k = 0; k < 16; ++k) {
+            const int sw = shw[k];
+            before += k < w ? sw : 0;
+            tile += sw;
+        }
+        if (r < nrows) col[r] = carry + before + incl - v;
+        carry += tile;
+        __syncthreads();
+    }
+    if (t == 0) totals[blockIdx.x] = carry;
 }
 
 // Stable scatter: items of a tile are ranked in (row, wave, lane) order, which
@@ -317,12 +311,21 @@ __global__ __launch_bounds__(1024) void k_radix_scan(int count, int* __restrict_
 __global__ __launch_bounds__(kBlk) void k_radix_scatter(
     int n, const unsigned long long* __restrict__ kin, const unsigned* __restrict__ vin,
     unsigned long long* __restrict__ kout, unsigned* __restrict__ vout, int shift, int nblk,
-    const int* __restrict__ offs) {
+    const int* __restrict__ offs, const int* __restrict__ digit_tot) {
     __shared__ int run[kSortBins];
     __shared__ int wc[4][kSortBins];
     const int t = threadIdx.x;
     const int w = t >> 6;
-    run[t] = offs[t * nblk + blockIdx.x];
+    {  // exclusive scan of the 256 digit totals -> start of each digit's bucket
+        const int v = digit_tot[t];
+        const int incl = wave_incl_scan(v);
+        if (lane_id() == 63) wc[0][w] = incl;
+        __syncthreads();
+        int before = 0;
+        for (int k = 0; k < w; ++k) before += wc[0][k];
+        run[t] = before + incl - v + offs[t * nblk + blockIdx.x];
+        __syncthreads();
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) wc[k][t] = 0;
     __syncthreads();
@@ -431,51 +434,44 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
     const bool group_start = valid && shi < gl;
     if (valid) gflag[i] = group_start ? 1 : 0;
     const bool over = leaf_start && mx + 1 > lmax && root->key_limited;
-    const unsigned long long bl = __ballot(leaf_start), bo = __ballot(over);
-    if (lane_id() == 0) {
-        if (bl) atomicAdd(&ctr->n_leaves, __popcll(bl));
-        if (bo) atomicAdd(&ctr->oversize, __popcll(bo));
-    }
 #pragma unroll
     for (int l = 0; l < kLevels; ++l) {
         const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
         if (lane_id() == 0) wcnt[w][l] = __popcll(bal);
     }
     {
-        const unsigned long long bal = __ballot(group_start);
-        if (lane_id() == 0) wcnt[w][kLevels] = __popcll(bal);
+        const unsigned long long bg = __ballot(group_start), bl = __ballot(leaf_start),
+                                 bo = __ballot(over);
+        if (lane_id() == 0) {
+            wcnt[w][kColGroup] = __popcll(bg);
+            wcnt[w][kColLeaf] = __popcll(bl);
+            wcnt[w][kColOver] = __popcll(bo);
+        }
     }
     __syncthreads();
     if (t < kCols)
         blkcnt[t * nblk + blockIdx.x] = wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
 }
 
-__global__ __launch_bounds__(1024) void k_level_scan(int n, int nblk, int* __restrict__ blk,
-                                                     LevelInfo* __restrict__ li, int node_cap,
-                                                     int* __restrict__ gstart,
-                                                     Counters* __restrict__ ctr) {
-    __shared__ int shw[17];
-    __shared__ int totals[kCols];
-    for (int l = 0; l < kCols; ++l) {
-        const int tot = block_scan_1024(blk + l * nblk, blk + l * nblk, nblk, shw);
-        if (threadIdx.x == 0) totals[l] = tot;
+// Level starts/counts, group and leaf totals from the column sums.
+__global__ void k_level_info(int n, const int* __restrict__ totals, LevelInfo* __restrict__ li,
+                             int node_cap, int* __restrict__ gstart, Counters* __restrict__ ctr) {
+    if (threadIdx.x != 0) return;
+    int run = 0, ml = 0;
+    for (int l = 0; l < kLevels; ++l) {
+        li->start[l] = run;
+        li->count[l] = totals[l];
+        run += totals[l];
+        if (totals[l] > 0) ml = l;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int run = 0, ml = 0;
-        for (int l = 0; l < kLevels; ++l) {
-            li->start[l] = run;
-            li->count[l] = totals[l];
-            run += totals[l];
-            if (totals[l] > 0) ml = l;
-        }
-        li->start[kLevels] = run;
-        li->total = run;
-        li->max_level = ml;
-        li->n_groups = totals[kLevels];
-        gstart[totals[kLevels]] = n;
-        if (run > node_cap) atomicOr(&ctr->overflow, 2);
-    }
+    li->start[kLevels] = run;
+    li->total = run;
+    li->max_level = ml;
+    li->n_groups = totals[kColGroup];
+    gstart[totals[kColGroup]] = n;
+    ctr->n_leaves = totals[kColLeaf];
+    ctr->oversize = totals[kColOver];
+    if (run > node_cap) atomicOr(&ctr->overflow, 2);
 }
 
 // Writes the node records: particle range, level, child range; and the leaf
@@ -552,126 +548,144 @@ __global__ __launch_bounds__(kBlk) void k_emit(
 
 
 // ------------------------------------------------------------------ up-sweep
-__global__ __launch_bounds__(kBlk) void k_upsweep(
-    int l, const LevelInfo* __restrict__ li, const RootInfo* __restrict__ root,
-    const int* __restrict__ pstart, const int* __restrict__ pend, const int* __restrict__ child,
-    const int* __restrict__ cend, const float4* __restrict__ spos, double* __restrict__ mom,
-    float4* __restrict__ mac, int2* __restrict__ link, float4* __restrict__ q0,
-    float4* __restrict__ q1, double inv_theta2, int quad, int n, Counters* __restrict__ ctr) {
-    const int cnt = li->count[l];
-    const int st = li->start[l];
-    const int total = li->total;
-    const double size = ldexp(root->size, -l);
-    const double r2open = size * size * inv_theta2;
-    for (int k = blockIdx.x * kBlk + threadIdx.x; k < cnt; k += gridDim.x * kBlk) {
-        const int id = st + k;
-        const int c0 = child[id];
-        // Structural checks: a malformed tree is reported, never dereferenced.
-        if (c0 < 0) {
-            const int a = pstart[id], b = pend[id];
-            if (a < 0 || b > n || a >= b) {
-                atomicOr(&ctr->overflow, 4);
-                link[id] = make_int2(~0, 0);
-                continue;
-            }
-        } else {
-            const int c1 = cend[id];
-            if (c0 <= id || c1 > total || c0 >= c1 || c1 - c0 > 8) {
-                atomicOr(&ctr->overflow, 4);
-                link[id] = make_int2(~0, 0);
-                continue;
-            }
+// Moments of node `id` (FP64): a leaf from its particles, an internal node
+// from its (already finished) children in child order.  Writes the FP64
+// record and the FP32 walk copies.
+__device__ void node_moments(int id, int n, int total, const RootInfo* root,
+                             const int* __restrict__ pstart, const int* __restrict__ pend,
+                             const int* child, const int* cend,
+                             const signed char* __restrict__ level,
+                             const float4* __restrict__ spos, double* mom, float4* mac,
+                             int2* link, float4* q0, float4* q1, double inv_theta2, int quad,
+                             Counters* ctr) {
+    const int c0 = child[id];
+    // Structural checks: a malformed tree is reported, never dereferenced.
+    if (c0 < 0) {
+        const int a = pstart[id], b = pend[id];
+        if (a < 0 || b > n || a >= b) {
+            atomicOr(&ctr->overflow, 4);
+            link[id] = make_int2(~0, 0);
+            return;
         }
-        double M = 0, cx = 0, cy = 0, cz = 0;
-        double Q[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz
-        if (c0 < 0) {
-            const int a = pstart[id], b = pend[id];
-            double sx = 0, sy = 0, sz = 0, ux = 0, uy = 0, uz = 0;
+    } else {
+        const int c1 = cend[id];
+        if (c0 <= id || c1 > total || c0 >= c1 || c1 - c0 > 8) {
+            atomicOr(&ctr->overflow, 4);
+            link[id] = make_int2(~0, 0);
+            return;
+        }
+    }
+    double M = 0, cx = 0, cy = 0, cz = 0;
+    double Q[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz
+    if (c0 < 0) {
+        const int a = pstart[id], b = pend[id];
+        double sx = 0, sy = 0, sz = 0, ux = 0, uy = 0, uz = 0;
+        for (int q = a; q < b; ++q) {
+            const float4 p = spos[q];
+            const double m = p.w;
+            M += m;
+            sx += m * p.x;
+            sy += m * p.y;
+            sz += m * p.z;
+            ux += p.x;
+            uy += p.y;
+            uz += p.z;
+        }
+        if (M > 0) {
+            cx = sx / M;
+            cy = sy / M;
+            cz = sz / M;
+        } else {
+            const double inv = 1.0 / (double)(b - a);
+            cx = ux * inv;
+            cy = uy * inv;
+            cz = uz * inv;
+        }
+        if (quad) {
             for (int q = a; q < b; ++q) {
                 const float4 p = spos[q];
                 const double m = p.w;
-                M += m;
-                sx += m * p.x;
-                sy += m * p.y;
-                sz += m * p.z;
-                ux += p.x;
-                uy += p.y;
-                uz += p.z;
-            }
-            if (M > 0) {
-                cx = sx / M;
-                cy = sy / M;
-                cz = sz / M;
-            } else {
-                const double inv = 1.0 / (double)(b - a);
-                cx = ux * inv;
-                cy = uy * inv;
-                cz = uz * inv;
-            }
-            if (quad) {
-                for (int q = a; q < b; ++q) {
-                    const float4 p = spos[q];
-                    const double m = p.w;
-                    const double dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    Q[0] += m * (3.0 * dx * dx - r2);
-                    Q[1] += m * (3.0 * dy * dy - r2);
-                    Q[2] += m * (3.0 * dz * dz - r2);
-                    Q[3] += m * (3.0 * dx * dy);
-                    Q[4] += m * (3.0 * dx * dz);
-                    Q[5] += m * (3.0 * dy * dz);
-                }
-            }
-        } else {
-            const int c1 = cend[id];
-            double sx = 0, sy = 0, sz = 0, ux = 0, uy = 0, uz = 0;
-            for (int c = c0; c < c1; ++c) {
-                const double* mc = mom + (size_t)c * 10;
-                M += mc[0];
-                sx += mc[0] * mc[1];
-                sy += mc[0] * mc[2];
-                sz += mc[0] * mc[3];
-                ux += mc[1];
-                uy += mc[2];
-                uz += mc[3];
-            }
-            if (M > 0) {
-                cx = sx / M;
-                cy = sy / M;
-                cz = sz / M;
-            } else {
-                const double inv = 1.0 / (double)(c1 - c0);
-                cx = ux * inv;
-                cy = uy * inv;
-                cz = uz * inv;
-            }
-            if (quad) {
-                for (int c = c0; c < c1; ++c) {
-                    const double* mc = mom + (size_t)c * 10;
-                    const double m = mc[0];
-                    const double dx = mc[1] - cx, dy = mc[2] - cy, dz = mc[3] - cz;
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    Q[0] += mc[4] + m * (3.0 * dx * dx - r2);
-                    Q[1] += mc[5] + m * (3.0 * dy * dy - r2);
-                    Q[2] += mc[6] + m * (3.0 * dz * dz - r2);
-                    Q[3] += mc[7] + m * (3.0 * dx * dy);
-                    Q[4] += mc[8] + m * (3.0 * dx * dz);
-                    Q[5] += mc[9] + m * (3.0 * dy * dz);
-                }
+                const double dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                Q[0] += m * (3.0 * dx * dx - r2);
+                Q[1] += m * (3.0 * dy * dy - r2);
+                Q[2] += m * (3.0 * dz * dz - r2);
+                Q[3] += m * (3.0 * dx * dy);
+                Q[4] += m * (3.0 * dx * dz);
+                Q[5] += m * (3.0 * dy * dz);
             }
         }
-        double* mo = mom + (size_t)id * 10;
-        mo[0] = M;
-        mo[1] = cx;
-        mo[2] = cy;
-        mo[3] = cz;
-#pragma unroll
-        for (int a = 0; a < 6; ++a) mo[4 + a] = Q[a];
-        mac[id] = make_float4((float)cx, (float)cy, (float)cz, (float)r2open);
-        link[id] = c0 < 0 ? make_int2(~pstart[id], pend[id]) : make_int2(c0, cend[id]);
-        q0[id] = make_float4((float)M, (float)Q[0], (float)Q[1], (float)Q[2]);
-        q1[id] = make_float4((float)Q[3], (float)Q[4], (float)Q[5], 0.f);
+    } else {
+        const int c1 = cend[id];
+        double sx = 0, sy = 0, sz = 0, ux = 0, uy = 0, uz = 0;
+        for (int c = c0; c < c1; ++c) {
+            const double* mc = mom + (size_t)c * 10;
+            M += mc[0];
+            sx += mc[0] * mc[1];
+            sy += mc[0] * mc[2];
+            sz += mc[0] * mc[3];
+            ux += mc[1];
+            uy += mc[2];
+            uz += mc[3];
+        }
+        if (M > 0) {
+            cx = sx / M;
+            cy = sy / M;
+            cz = sz / M;
+        } else {
+            const double inv = 1.0 / (double)(c1 - c0);
+            cx = ux * inv;
+            cy = uy * inv;
+            cz = uz * inv;
+        }
+        if (quad) {
+            for (int c = c0; c < c1; ++c) {
+                const double* mc = mom + (size_t)c * 10;
+                const double m = mc[0];
+                const double dx = mc[1] - cx, dy = mc[2] - cy, dz = mc[3] - cz;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                Q[0] += mc[4] + m * (3.0 * dx * dx - r2);
+                Q[1] += mc[5] + m * (3.0 * dy * dy - r2);
+                Q[2] += mc[6] + m * (3.0 * dz * dz - r2);
+                Q[3] += mc[7] + m * (3.0 * dx * dy);
+                Q[4] += mc[8] + m * (3.0 * dx * dz);
+                Q[5] += mc[9] + m * (3.0 * dy * dz);
+            }
+        }
     }
+    double* mo = mom + (size_t)id * 10;
+    mo[0] = M;
+    mo[1] = cx;
+    mo[2] = cy;
+    mo[3] = cz;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) mo[4 + a] = Q[a];
+    const double size = ldexp(root->size, -(int)level[id]);
+    const double r2open = size * size * inv_theta2;
+    mac[id] = make_float4((float)cx, (float)cy, (float)cz, (float)r2open);
+    link[id] = c0 < 0 ? make_int2(~pstart[id], pend[id]) : make_int2(c0, cend[id]);
+    q0[id] = make_float4((float)M, (float)Q[0], (float)Q[1], (float)Q[2]);
+    q1[id] = make_float4((float)Q[3], (float)Q[4], (float)Q[5], 0.f);
+}
+
+// Level-synchronous up-sweep: one launch per level, deepest level first, so
+// every child is finished (and visible at the kernel boundary) before its
+// parent is computed.  (A single-launch version with per-node arrival
+// counters needs a device-scope release/acquire per node, which on this GPU
+// writes back / invalidates the XCD-local L2 and was 5x slower.)
+__global__ __launch_bounds__(kBlk) void k_upsweep(
+    int l, const LevelInfo* __restrict__ li, const RootInfo* __restrict__ root,
+    const int* __restrict__ pstart, const int* __restrict__ pend, const int* child,
+    const int* cend, const signed char* __restrict__ level, const float4* __restrict__ spos,
+    double* mom, float4* mac, int2* link, float4* q0, float4* q1, double inv_theta2, int quad,
+    int n, Counters* ctr) {
+    if (ctr->overflow & 6) return;
+    const int cnt = li->count[l];
+    const int st = li->start[l];
+    const int total = li->total;
+    for (int k = blockIdx.x * kBlk + threadIdx.x; k < cnt; k += gridDim.x * kBlk)
+        node_moments(st + k, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
+                     link, q0, q1, inv_theta2, quad, ctr);
 }
 
 // ------------------------------------------------------------------ walk
@@ -976,6 +990,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_TRY(w->vals0.reserve((size_t)n * 4));
     NB_TRY(w->vals1.reserve((size_t)n * 4));
     NB_TRY(w->hist.reserve((size_t)nsort * kSortBins * 4));
+    NB_TRY(w->totals.reserve((size_t)std::max(kSortBins, kCols) * 4));
     NB_TRY(w->spos.reserve((size_t)n * 16));
     NB_TRY(w->lvl_leaf.reserve((size_t)n));
     NB_TRY(w->lvl_sh.reserve((size_t)n));
@@ -1016,10 +1031,10 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
         const int shift = 8 * p;
         hipLaunchKernelGGL(k_radix_hist, dim3(nsort), dim3(kBlk), 0, s, n, k0, shift, nsort,
                            w->hist.as<int>());
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, nsort * kSortBins,
-                           w->hist.as<int>());
+        hipLaunchKernelGGL(k_col_scan, dim3(kSortBins), dim3(1024), 0, s, nsort,
+                           w->hist.as<int>(), w->totals.as<int>());
         hipLaunchKernelGGL(k_radix_scatter, dim3(nsort), dim3(kBlk), 0, s, n, k0, v0, k1, v1,
-                           shift, nsort, w->hist.as<int>());
+                           shift, nsort, w->hist.as<int>(), w->totals.as<int>());
         std::swap(k0, k1);
         std::swap(v0, v1);
     }
@@ -1033,7 +1048,9 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->lvl_leaf.as<signed char>(), w->lvl_sh.as<signed char>(),
                        w->gflag.as<unsigned char>(), w->glev.as<signed char>(), nblk,
                        w->blkcnt.as<int>(), ctr);
-    hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(1024), 0, s, n, nblk, w->blkcnt.as<int>(), li,
+    hipLaunchKernelGGL(k_col_scan, dim3(kCols), dim3(1024), 0, s, nblk, w->blkcnt.as<int>(),
+                       w->totals.as<int>());
+    hipLaunchKernelGGL(k_level_info, dim3(1), dim3(64), 0, s, n, w->totals.as<int>(), li,
                        (int)ncap, w->gstart.as<int>(), ctr);
     hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(kBlk), 0, s, n, w->lvl_leaf.as<signed char>(),
                        w->lvl_sh.as<signed char>(), nblk, w->blkcnt.as<int>(), li, (int)ncap,
@@ -1046,10 +1063,10 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     for (int l = kLevels - 1; l >= 0; --l) {
         hipLaunchKernelGGL(k_upsweep, dim3(kUpsweepGrid), dim3(kBlk), 0, s, l, li, root,
                            w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
-                           w->n_cend.as<int>(), w->spos.as<float4>(), w->n_mom.as<double>(),
-                           w->w_mac.as<float4>(), w->w_link.as<int2>(), w->w_q0.as<float4>(),
-                           w->w_q1.as<float4>(), inv_theta2, multipole == NB_QUADRUPOLE ? 1 : 0,
-                           n, ctr);
+                           w->n_cend.as<int>(), w->n_level.as<signed char>(),
+                           w->spos.as<float4>(), w->n_mom.as<double>(), w->w_mac.as<float4>(),
+                           w->w_link.as<int2>(), w->w_q0.as<float4>(), w->w_q1.as<float4>(),
+                           inv_theta2, multipole == NB_QUADRUPOLE ? 1 : 0, n, ctr);
     }
     NB_LAUNCH_CHECK();
     if (ev) NB_HIP(hipEventRecord(ev[6], s));
@@ -1059,7 +1076,9 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
         int dev = 0, cus = 0;
         NB_HIP(hipGetDevice(&dev));
         NB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        w->walk_blocks = std::max(1, cus) * kWalkBlocksPerCU;
+        int bpc = kWalkBlocksPerCU;
+        if (const char* e = std::getenv("NB_WALK_BPC")) bpc = std::max(1, std::min(64, std::atoi(e)));
+        w->walk_blocks = std::max(1, cus) * bpc;
     }
     const int wblocks = std::min(ngroups, w->walk_blocks);
     NB_TRY(w->walk_scratch.reserve((size_t)w->walk_blocks * 2 * kWalkFcap * sizeof(int2)));
