@@ -255,9 +255,11 @@ int nb_ensemble_run(nb_ensemble* e, int steps, int use_graph);
 /* The same steps with one launch sequence per system (baseline for the
  * batched path; same per-system arithmetic, hence the same bits). */
 int nb_ensemble_run_unbatched(nb_ensemble* e, int steps);
-/* Autotuner: time the tile shapes per size bucket on this device and store
- * the choice (also written to `path` if not NULL).  nb_ensemble_tuning
- * returns the per-bucket shape codes (bucket b covers N in [2^(b+5), 2^(b+6))). */
+/* Autotuner: time the tile shapes (0: 64 targets x 4 source slices, 1: 128 x 2,
+ * 2: 256 x 1 per 256-lane workgroup) on every occupied size bucket of this
+ * ensemble and keep the fastest (also written to `path` if not NULL).
+ * nb_ensemble_tuning returns the 8 per-bucket shape codes; bucket b covers
+ * N in [2^(b+6), 2^(b+7)), bucket 0 everything below 128, bucket 7 from 8192. */
 int nb_ensemble_autotune(nb_ensemble* e, const char* path);
 int nb_ensemble_load_tuning(nb_ensemble* e, const char* path);
 int nb_ensemble_tuning(const nb_ensemble* e, int32_t* shape_codes, int n_buckets);

@@ -1,0 +1,300 @@
+// nbody.hpp -- C++ drop-in adapter: the reference's force-calculator and
+// integrator classes, with both of its call signatures, implemented over the
+// C ABI of libnbody_amd.so (include/nbody_amd.h).
+//
+//  * CPU-style API (CPU/include/force_calculator.hpp:13-31, CPU/include/integrator.hpp:5-26):
+//      calculator.computeAccelerations(std::vector<Particle>&)
+//      integrator.step(std::vector<Particle>&, ForceCalculator&, double dt)
+//    The host vector is uploaded, the work runs on the GPU, results come back
+//    into the vector (the reference's semantics: the call returns with the
+//    vector updated).
+//  * CUDA-style API (CUDA/include/force_calculator.hpp:14-30, CUDA/include/integrator.hpp:7-17,
+//    CUDA/include/particle_data.hpp:6-73):
+//      calculator.computeAccelerations(ParticleArrays&)
+//      integrator.step(ForceCalculator&, ParticleArrays&, double dt)
+//    State stays resident on the device; allocateParticles / copyParticlesToDevice /
+//    copyParticlesToHost / freeParticles move it.
+//
+// Errors: a non-zero C-ABI status throws nbody::Error (the reference calls
+// exit() or assert()).  G defaults to the reference's 4*pi^2
+// (CPU/include/common.hpp:28); setG() changes it.
+#pragma once
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nbody_amd.h"
+
+namespace constant {
+constexpr double PI = 3.14159265358979323846;
+constexpr double G = 4.0 * PI * PI;
+}  // namespace constant
+
+// Same layout and operator rounding as CPU/include/particle.hpp:5-93.
+struct Vector3D {
+    double x, y, z;
+    Vector3D() : x(0), y(0), z(0) {}
+    Vector3D(double x_, double y_, double z_) : x(x_), y(y_), z(z_) {}
+    double norm() const { return std::sqrt(x * x + y * y + z * z); }
+    Vector3D operator+(const Vector3D& o) const { return {x + o.x, y + o.y, z + o.z}; }
+    Vector3D operator-(const Vector3D& o) const { return {x - o.x, y - o.y, z - o.z}; }
+    Vector3D operator*(double s) const { return {x * s, y * s, z * s}; }
+    Vector3D& operator+=(const Vector3D& o) {
+        x += o.x;
+        y += o.y;
+        z += o.z;
+        return *this;
+    }
+    double dot(const Vector3D& o) const { return x * o.x + y * o.y + z * o.z; }
+};
+
+struct Particle {
+    double mass;
+    Vector3D position, velocity, acceleration;
+    std::string name;
+    Particle() : mass(1) {}
+    Particle(double m, const Vector3D& p, const Vector3D& v, const std::string& n = "")
+        : mass(m), position(p), velocity(v), name(n) {}
+};
+
+// CPU/include/common.hpp:40-51
+struct TimingStatistics {
+    std::string name;
+    double total_time = 0.0;
+    size_t calls = 0;
+    double average_time() const { return calls > 0 ? total_time / calls : 0.0; }
+    void reset() {
+        total_time = 0.0;
+        calls = 0;
+    }
+};
+
+// Device state (ParticleArrays, CUDA/include/particle_data.hpp:6-43).
+using ParticleArrays = nb_particles;
+
+namespace nbody {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+
+inline void check(int rc, const char* where) {
+    if (rc != NB_OK) throw Error(rc, std::string(where) + ": " + nb_last_error());
+}
+
+// Host <-> device staging for the CPU-style API.
+struct HostSoA {
+    std::vector<double> x, y, z, vx, vy, vz, ax, ay, az, m;
+    void from(const std::vector<Particle>& p) {
+        const size_t n = p.size();
+        for (auto* v : {&x, &y, &z, &vx, &vy, &vz, &ax, &ay, &az, &m}) v->resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            x[i] = p[i].position.x;
+            y[i] = p[i].position.y;
+            z[i] = p[i].position.z;
+            vx[i] = p[i].velocity.x;
+            vy[i] = p[i].velocity.y;
+            vz[i] = p[i].velocity.z;
+            m[i] = p[i].mass;
+        }
+    }
+};
+
+// Device mirror of a host particle vector, reused across calls.
+struct DeviceMirror {
+    nb_particles d{};
+    HostSoA h;
+    ~DeviceMirror() { nb_particles_free(&d); }
+    void upload(const std::vector<Particle>& p) {
+        if (d.n != (int64_t)p.size()) {
+            check(nb_particles_free(&d), "freeParticles");
+            check(nb_particles_alloc(&d, (int64_t)p.size()), "allocateParticles");
+        }
+        h.from(p);
+        check(nb_particles_upload(&d, h.x.data(), h.y.data(), h.z.data(), h.vx.data(), h.vy.data(),
+                                  h.vz.data(), h.m.data()),
+              "copyParticlesToDevice");
+    }
+    void download(std::vector<Particle>& p, bool state) {
+        check(nb_particles_download(&d, state ? h.x.data() : nullptr, state ? h.y.data() : nullptr,
+                                    state ? h.z.data() : nullptr, state ? h.vx.data() : nullptr,
+                                    state ? h.vy.data() : nullptr, state ? h.vz.data() : nullptr,
+                                    h.ax.data(), h.ay.data(), h.az.data(), nullptr),
+              "copyParticlesToHost");
+        for (size_t i = 0; i < p.size(); ++i) {
+            if (state) {
+                p[i].position = Vector3D(h.x[i], h.y[i], h.z[i]);
+                p[i].velocity = Vector3D(h.vx[i], h.vy[i], h.vz[i]);
+            }
+            p[i].acceleration = Vector3D(h.ax[i], h.ay[i], h.az[i]);
+        }
+    }
+};
+
+}  // namespace nbody
+
+// CUDA/include/particle_data.hpp:45-73
+inline void allocateParticles(ParticleArrays& p, size_t n) {
+    nbody::check(nb_particles_alloc(&p, (int64_t)n), "allocateParticles");
+}
+inline void freeParticles(ParticleArrays& p) { nbody::check(nb_particles_free(&p), "freeParticles"); }
+inline void copyParticlesToDevice(ParticleArrays& p, const double* x, const double* y,
+                                  const double* z, const double* vx, const double* vy,
+                                  const double* vz, const double* m) {
+    nbody::check(nb_particles_upload(&p, x, y, z, vx, vy, vz, m), "copyParticlesToDevice");
+}
+inline void copyParticlesToHost(const ParticleArrays& p, double* x, double* y, double* z,
+                                double* vx, double* vy, double* vz, double* ax, double* ay,
+                                double* az, double* m) {
+    nbody::check(nb_particles_download(&p, x, y, z, vx, vy, vz, ax, ay, az, m),
+                 "copyParticlesToHost");
+}
+
+// ForceCalculator base: CPU/include/force_calculator.hpp:13-31 and
+// CUDA/include/force_calculator.hpp:14-30 in one class.
+class ForceCalculator {
+   protected:
+    double softening_squared;
+    double softening_;
+    double G_ = constant::G;
+    nb_force* h_ = nullptr;
+    nbody::DeviceMirror mirror_;
+    mutable TimingStatistics stats_;
+
+    void sync_stats() const {
+        double t[4];
+        if (h_ && nb_force_timing(h_, t) == NB_OK) {
+            stats_.total_time = t[0];
+            stats_.calls = (size_t)t[3];
+        }
+    }
+
+   public:
+    ForceCalculator() : softening_squared(0.0), softening_(0.0) {}
+    explicit ForceCalculator(double softening)
+        : softening_squared(softening * softening), softening_(softening) {}
+    virtual ~ForceCalculator() { nb_force_destroy(h_); }
+    ForceCalculator(const ForceCalculator&) = delete;
+    ForceCalculator& operator=(const ForceCalculator&) = delete;
+
+    virtual void computeAccelerations(std::vector<Particle>& particles) {
+        mirror_.upload(particles);
+        computeAccelerations(mirror_.d);
+        mirror_.download(particles, false);
+    }
+    virtual void computeAccelerations(ParticleArrays& particles) {
+        nbody::check(nb_force_compute(h_, &particles), "computeAccelerations");
+    }
+    virtual std::string name() const { return nb_force_name(h_); }
+    virtual const TimingStatistics& getTimeStatistics() const {
+        sync_stats();
+        return stats_;
+    }
+    nb_force* handle() const { return h_; }
+};
+
+// ForceCalculatorNaive(double softening): CPU/include/force_calculator_naive.hpp:12
+// (FP64, bit-identical to the reference CPU) or the FP32 tiled kernel.
+class ForceCalculatorNaive : public ForceCalculator {
+   public:
+    explicit ForceCalculatorNaive(double softening = 0.0, int precision = NB_DIRECT_F64_EXACT,
+                                  double G = constant::G)
+        : ForceCalculator(softening) {
+        G_ = G;
+        nbody::check(nb_force_create_direct(&h_, precision, G, softening), "ForceCalculatorNaive");
+        nb_force_set_timing(h_, 1);
+        stats_.name = "Naive Force Calculation";
+    }
+};
+
+// BarnesHutForceCalculator(double softening, double theta, int leaf_capacity = 8):
+// CUDA/include/force_calculator_barnes_hut.hpp:56.
+class BarnesHutForceCalculator : public ForceCalculator {
+   public:
+    BarnesHutForceCalculator(double softening, double theta, int leaf_capacity = 8,
+                             int multipole = NB_QUADRUPOLE, double G = constant::G)
+        : ForceCalculator(softening) {
+        G_ = G;
+        nbody::check(nb_force_create_bh(&h_, G, softening, theta, leaf_capacity, multipole),
+                     "BarnesHutForceCalculator");
+        nb_force_set_timing(h_, 1);
+        stats_.name = "Barnes-Hut Force Calculation";
+    }
+    nb_bh_stats stats() const {
+        nb_bh_stats s{};
+        nbody::check(nb_force_bh_stats(h_, &s), "BarnesHutForceCalculator::stats");
+        return s;
+    }
+    // CUDA/include/force_calculator_barnes_hut.hpp:120-125
+    double getTotalComputeTime() const {
+        double t[4];
+        nbody::check(nb_force_timing(h_, t), "timing");
+        return t[0];
+    }
+    double getBuildTreeTime() const {
+        double t[4];
+        nbody::check(nb_force_timing(h_, t), "timing");
+        return t[1];
+    }
+    double getTraversalTime() const {
+        double t[4];
+        nbody::check(nb_force_timing(h_, t), "timing");
+        return t[2];
+    }
+    void resetTimeStatistics() { nb_force_reset_timing(h_); }
+};
+
+// ForceCalculatorBarnesHutOptimized(double softening, double theta, int leaf_capacity):
+// CPU/include/force_calculator_barnes_hut_optimized.hpp:40.  Same tree as the
+// reference CPU; forces from the GPU group walk.
+class ForceCalculatorBarnesHutOptimized : public BarnesHutForceCalculator {
+   public:
+    ForceCalculatorBarnesHutOptimized(double softening, double theta, int leaf_capacity)
+        : BarnesHutForceCalculator(softening, theta, leaf_capacity) {}
+};
+
+// TimeIntegrator base with both step() signatures
+// (CPU/include/integrator.hpp:16-20, CUDA/include/integrator.hpp:10-14).
+class TimeIntegrator {
+   protected:
+    nb_integrator* h_ = nullptr;
+    nbody::DeviceMirror mirror_;
+
+   public:
+    explicit TimeIntegrator(int kind) {
+        nbody::check(nb_integrator_create(&h_, kind), "TimeIntegrator");
+    }
+    virtual ~TimeIntegrator() { nb_integrator_destroy(h_); }
+    TimeIntegrator(const TimeIntegrator&) = delete;
+    TimeIntegrator& operator=(const TimeIntegrator&) = delete;
+
+    virtual void step(std::vector<Particle>& particles, ForceCalculator& calculator, double dt) {
+        mirror_.upload(particles);
+        nb_integrator_reset(h_);  // host state may have changed since the last call
+        step(calculator, mirror_.d, dt);
+        mirror_.download(particles, true);
+    }
+    virtual void step(ForceCalculator& calculator, ParticleArrays& particles, double dt) {
+        nbody::check(nb_integrator_step(h_, calculator.handle(), &particles, dt), "step");
+    }
+    // Several steps without host round trips (device state).
+    void run(ForceCalculator& calculator, ParticleArrays& particles, double dt, int steps) {
+        nbody::check(nb_integrator_run(h_, calculator.handle(), &particles, dt, steps), "run");
+    }
+    virtual std::string name() const { return nb_integrator_name(h_); }
+};
+
+// CPU/include/integrator_leapfrog.hpp, CUDA/include/integrator_leapfrog.hpp
+class LeapfrogIntegrator : public TimeIntegrator {
+   public:
+    LeapfrogIntegrator() : TimeIntegrator(NB_LEAPFROG) {}
+};
+
+// CPU/include/integrator_rk4.hpp
+class RK4Integrator : public TimeIntegrator {
+   public:
+    RK4Integrator() : TimeIntegrator(NB_RK4) {}
+};

@@ -64,6 +64,7 @@ class _Diag(C.Structure):
 
 
 _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _Dn = C.c_void_p  # nullable double*
 _vp = C.c_void_p
 
@@ -110,6 +111,15 @@ def _load():
         "nb_ic_plummer_disk": ([C.c_int64, C.c_uint64] + [C.c_double] * 6 + [_D] * 7, C.c_int),
         "nb_file_read": ([C.c_char_p, C.POINTER(C.c_int64)] + [_Dn] * 7 + [C.c_int64], C.c_int),
         "nb_file_write": ([C.c_char_p, C.c_int64] + [_D] * 7, C.c_int),
+        "nb_ensemble_create": ([C.POINTER(_vp), C.c_int, _I64, _D, _D, _D], C.c_int),
+        "nb_ensemble_destroy": ([_vp], C.c_int),
+        "nb_ensemble_upload": ([_vp] + [_D] * 7, C.c_int),
+        "nb_ensemble_download": ([_vp] + [_Dn] * 9, C.c_int),
+        "nb_ensemble_run": ([_vp, C.c_int, C.c_int], C.c_int),
+        "nb_ensemble_run_unbatched": ([_vp, C.c_int], C.c_int),
+        "nb_ensemble_autotune": ([_vp, C.c_char_p], C.c_int),
+        "nb_ensemble_load_tuning": ([_vp, C.c_char_p], C.c_int),
+        "nb_ensemble_tuning": ([_vp, _vp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -393,3 +403,55 @@ def write_particles(path, m, x, y, z, vx=None, vy=None, vz=None):
     zero = np.zeros(n)
     arrs = [_f64(a) for a in (m, x, y, z)] + [zero if v is None else _f64(v) for v in (vx, vy, vz)]
     _check(lib.nb_file_write(os.fsencode(path), n, *arrs))
+
+
+# --------------------------------------------------------------------------- ensembles
+class Ensemble:
+    """Many independent systems advanced together (one persistent force launch
+    per step over all systems, optional HIP-graph replay)."""
+
+    def __init__(self, sizes, G=1.0, softening=0.01, dt=1e-3):
+        sizes = np.asarray(sizes, dtype=np.int64)
+        self.S = len(sizes)
+        self.offsets = np.zeros(self.S + 1, np.int64)
+        self.offsets[1:] = np.cumsum(sizes)
+        self.N = int(self.offsets[-1])
+        per = lambda v: np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (self.S,)))
+        h = C.c_void_p()
+        _check(lib.nb_ensemble_create(C.byref(h), self.S, self.offsets, per(G), per(softening),
+                                      per(dt)))
+        self._h = h
+
+    def upload(self, m, x, y, z, vx, vy, vz):
+        _check(lib.nb_ensemble_upload(self._h, *[_f64(a) for a in (m, x, y, z, vx, vy, vz)]))
+
+    def download(self):
+        names = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az")
+        out = {k: np.empty(self.N) for k in names}
+        _check(lib.nb_ensemble_download(self._h, *[out[k].ctypes.data for k in names]))
+        return out
+
+    def run(self, steps, use_graph=True):
+        _check(lib.nb_ensemble_run(self._h, int(steps), int(use_graph)))
+
+    def run_unbatched(self, steps):
+        _check(lib.nb_ensemble_run_unbatched(self._h, int(steps)))
+
+    def autotune(self, path=None):
+        _check(lib.nb_ensemble_autotune(self._h, None if path is None else os.fsencode(path)))
+
+    def load_tuning(self, path):
+        _check(lib.nb_ensemble_load_tuning(self._h, os.fsencode(path)))
+
+    def tuning(self):
+        out = np.zeros(8, np.int32)
+        _check(lib.nb_ensemble_tuning(self._h, out.ctypes.data, 8))
+        return out
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib.nb_ensemble_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
