@@ -10,6 +10,9 @@ evaluation (bbox, keys, radix sort, build, up-sweep, walk) + FP64 kick/drift.
 W warm-up steps, then exactly K timed steps between device synchronisations;
 the state is resident in HBM throughout.
 
+--workload direct / ensemble time the other two direct-sum workloads
+(N = 262,144 Plummer + disk with RK4; 4096-system ensemble) instead.
+
 Prints ONE JSON line.  Besides the contract fields it carries:
   roofline      the walk kernel (dominant): algorithmic FP32 FLOPs per launch
                 (20 per particle-particle, 60 per particle-cell quadrupole
@@ -84,25 +87,139 @@ def cpu_baseline(pl, n_sample):
             "seconds": (ms["build_ms"] + ms["walk_ms"] + ms["destroy_ms"]) * 1e-3}
 
 
+def bench_direct(args):
+    """Direct tiled FP32 all-pairs with FP64 RK4 state: N = 262,144 Plummer
+    (90% mass) + cold exponential disk (10%), eps = 0.01, dt = 5e-4, 20 RK4
+    steps (80 force evaluations); error and speed against the FP64 kernel."""
+    import nbody_amd as nb
+
+    n = args.n if args.n != N_DEFAULT else 262144
+    K = args.steps if args.steps_set else 20
+    W = args.warmup
+    d = nb.ic_plummer_disk(n, 5)
+    f = nb.ForceCalculatorNaive(0.01, "f32", G=1.0)
+    p = nb.Particles.from_arrays(d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"])
+    e0 = nb.diagnostics(p, 1.0, 0.01)
+    it = nb.RK4Integrator()
+    it.run(f, p, 5e-4, W)
+    nb.synchronize()
+    f.set_timing(True)
+    f.reset_timing()
+    t0 = time.perf_counter()
+    it.run(f, p, 5e-4, K)
+    nb.synchronize()
+    elapsed = time.perf_counter() - t0
+    tm = f.timing()
+    e1 = nb.diagnostics(p, 1.0, 0.01)
+    inter = 4 * K * n * (n - 1)
+    kern_ms = tm["walk_ms"] / tm["calls"]
+    achieved = FLOP_PP * n * (n - 1) / (kern_ms * 1e-3) / 1e12
+    # accuracy of one evaluation against the FP64 kernel on the final state
+    f.computeAccelerations(p)
+    a32 = np.stack(p.acc(), 1)
+    f64 = nb.ForceCalculatorNaive(0.01, "f64", G=1.0)
+    f64.set_timing(True)
+    f64.computeAccelerations(p)
+    a64 = np.stack(p.acc(), 1)
+    t64 = f64.timing()["total_ms"]
+    err = np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1)
+    E0, E1 = e0["ke"] + e0["pe_soft"], e1["ke"] + e1["pe_soft"]
+    return {
+        "metric": "interactions/s (direct) & particle-steps/s (BH, N=1M); serve p50/p99 latency",
+        "value": inter / elapsed, "unit": "interactions/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": elapsed / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": (inter / elapsed) / 4.12e9, "dtype": "f32",
+        "data": "synthetic (seeded Plummer + exponential disk, Philox4x32-10 streams)",
+        "config": {"workload": f"direct tiled all-pairs FP32 forces, FP64 RK4 state, N={n} "
+                               f"Plummer (90%) + cold exponential disk (10%, R_d=1, z_d=0.05), "
+                               f"eps=0.01, dt=5e-4, {K} steps (4 evaluations per step)",
+                   "baseline_published": "4.12e9 interactions/s: reference FP64 GPU direct, "
+                                         "N=100,000 (CUDA/outputs/output1_1.txt:130)"},
+        "roofline": {"kernel": "k_direct_f32", "bound": "valu", "achieved": round(achieved, 2),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "launch_ms": round(kern_ms, 4), "flops_per_launch": FLOP_PP * n * (n - 1)},
+        "accuracy": {"median_rel_vs_f64": float(np.median(err)),
+                     "p99_rel_vs_f64": float(np.percentile(err, 99)),
+                     "energy_drift_rel": float(abs(E1 - E0) / abs(E0))},
+        "f64_direct": {"ms_per_eval": t64, "interactions_per_s": n * (n - 1) / (t64 * 1e-3)},
+    }
+
+
+def bench_ensemble(args):
+    """4096 independent Plummer systems, N_k log-uniform in [64, 8192] and scale
+    radius uniform in [0.5, 2] (seeded), direct FP32 forces with FP64 leapfrog,
+    eps = 0.01, dt = 1e-3, 200 steps captured in one HIP graph; tile shapes per
+    size bucket from the autotuner.  Compared with one-system-at-a-time
+    launches of the same kernels (timed on a few steps)."""
+    import nbody_amd as nb
+
+    S = 4096
+    K = args.steps if args.steps_set else 200
+    rng = np.random.Generator(np.random.Philox(4))
+    sizes = np.rint(np.exp(rng.uniform(np.log(64), np.log(8192), S))).astype(np.int64)
+    radii = rng.uniform(0.5, 2.0, S)
+    parts = [nb.ic_plummer(int(n), 1000 + k, a=float(a)) for k, (n, a) in enumerate(zip(sizes, radii))]
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    ntot = int(sizes.sum())
+    e = nb.Ensemble(sizes, G=1.0, softening=0.01, dt=1e-3)
+    e.upload(cat["m"], cat["x"], cat["y"], cat["z"], cat["vx"], cat["vy"], cat["vz"])
+    e.autotune()
+    e.run(2, use_graph=True)  # warm-up: prime + capture a 2-step graph
+    nb.synchronize()
+    e.run(K, use_graph=True)  # captures the K-step graph (not timed)
+    nb.synchronize()
+    t0 = time.perf_counter()
+    e.run(K, use_graph=True)
+    nb.synchronize()
+    t_b = time.perf_counter() - t0
+    ku = 3
+    t0 = time.perf_counter()
+    e.run_unbatched(ku)
+    nb.synchronize()
+    t_u = (time.perf_counter() - t0) / ku
+    pairs = float(np.sum(sizes * (sizes - 1)))
+    value = ntot * K / t_b
+    return {
+        "metric": "interactions/s (direct) & particle-steps/s (BH, N=1M); serve p50/p99 latency",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": 2,
+        "ms_per_step": t_b / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Plummer spheres)",
+        "config": {"workload": f"ensemble of {S} Plummer systems, N log-uniform [64, 8192] "
+                               f"(total {ntot}), a ~ U[0.5, 2], direct FP32 + FP64 leapfrog, "
+                               f"eps=0.01, dt=1e-3, {K} steps in one HIP graph, autotuned shapes",
+                   "shapes_per_bucket": e.tuning().tolist()},
+        "interactions_per_s": pairs * K / t_b,
+        "unbatched": {"ms_per_step": t_u * 1e3, "particle_steps_per_s": ntot / t_u,
+                      "speedup_batched": (ntot * K / t_b) / (ntot / t_u)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=N_DEFAULT)
     ap.add_argument("--leaf-cap", type=int, default=CAP)
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--workload", choices=("bh", "direct", "ensemble"), default="bh")
     args = ap.parse_args()
+    args.steps_set = args.steps is not None
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: every box has one MI355X and the "
                                    "engine has no multi-GPU path; run with --gpus 1"}), flush=True)
         sys.exit(2)
+    if args.workload != "bh":
+        out = bench_direct(args) if args.workload == "direct" else bench_ensemble(args)
+        print(json.dumps(out), flush=True)
+        return
 
     import nbody_amd as nb
 
-    n, K, W = args.n, args.steps, args.warmup
+    n, K, W = args.n, (args.steps if args.steps_set else 10), args.warmup
     log(f"[bench] generating Plummer N={n}")
     pl = nb.ic_plummer(n, SEED)
     f = nb.BarnesHutForceCalculator(EPS, THETA, args.leaf_cap, "quadrupole", G=1.0)

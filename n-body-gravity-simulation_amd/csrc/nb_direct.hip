@@ -13,6 +13,7 @@
 //  * k_direct_f32: FP32 with v_rsq_f32, 3 sub + 3 fma + rsq + 3 mul + 3 fma
 //    per interaction, R targets per lane to amortise the LDS broadcast.
 #include <climits>
+#include <cstdlib>
 
 #include "nb_internal.hpp"
 
@@ -209,6 +210,10 @@ int direct_eval(int precision, double G, double soft2, int64_t n64, const double
                                    n, x, y, z, m, G, soft2, ax, ay, az);
             break;
         case NB_DIRECT_F32: {
+            if (ws.rows == 0) {  // targets per lane: NB_DIRECT_ROWS overrides the default
+                ws.rows = 1;
+                if (const char* e = std::getenv("NB_DIRECT_ROWS")) ws.rows = std::atoi(e);
+            }
             NB_TRY(ws.pack.reserve((size_t)n * sizeof(float4)));
             float4* pm = ws.pack.as<float4>();
             hipLaunchKernelGGL(k_pack_f32, dim3(grid), dim3(kDirectBlock), 0, s, n, x, y, z, m, G,

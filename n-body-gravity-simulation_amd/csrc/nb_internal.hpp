@@ -58,7 +58,7 @@ inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block
 // ------------------------------------------------------------------ direct sum
 struct DirectWS {
     DevBuf pack;  // float4 (x, y, z, G*m) for the FP32 kernel
-    int rows = 1; // targets per thread for the FP32 kernel (autotuned)
+    int rows = 0; // targets per lane of the FP32 kernel (0: not chosen yet)
 };
 
 // Launches the direct-sum force for positions (x,y,z) and masses m, writing

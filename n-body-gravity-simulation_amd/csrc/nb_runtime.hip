@@ -71,13 +71,15 @@ int force_eval(nb_force* f, int64_t n, const double* m, const double* x, const d
         f->ev_pending.push_back((int)f->ev_used);
         ev = &f->ev_pool[f->ev_used];
         f->ev_used = need;
-        NB_HIP(hipEventRecord(ev[0], f->stream));
+        // A direct sum has no tree stages: ev[0..kStages-1] are recorded back to
+        // back before it, so its whole time is booked as the last ("walk") stage.
+        const int first = f->kind == 0 ? kStages - 1 : 0;
+        for (int k = 0; k <= first; ++k) NB_HIP(hipEventRecord(ev[k], f->stream));
     }
     if (f->kind == 0) {
         NB_TRY(direct_eval(f->precision, f->G, f->soft2, n, m, x, y, z, ax, ay, az, f->direct,
                            f->stream));
-        if (ev)  // a direct sum has no tree stages: all time is booked as "walk"
-            for (int k = 1; k <= kStages; ++k) NB_HIP(hipEventRecord(ev[k], f->stream));
+        if (ev) NB_HIP(hipEventRecord(ev[kStages], f->stream));
     } else {
         NB_TRY(bh_eval(f->bh, f->G, f->soft2, f->theta, f->leaf_cap, f->multipole, n, m, x, y, z,
                        ax, ay, az, f->stream, ev));
