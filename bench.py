@@ -10,8 +10,9 @@ evaluation (bbox, keys, radix sort, build, up-sweep, walk) + FP64 kick/drift.
 W warm-up steps, then exactly K timed steps between device synchronisations;
 the state is resident in HBM throughout.
 
---workload direct / ensemble time the other two direct-sum workloads
-(N = 262,144 Plummer + disk with RK4; 4096-system ensemble) instead.
+--workload direct / ensemble / serve time the other workloads instead
+(N = 262,144 Plummer + disk with RK4; 4096-system ensemble; Poisson-arrival
+serving sweep with p50/p99 latency).
 
 Prints ONE JSON line.  Besides the contract fields it carries:
   roofline      the walk kernel (dominant): algorithmic FP32 FLOPs per launch
@@ -195,6 +196,68 @@ def bench_ensemble(args):
     }
 
 
+def bench_serve(args):
+    """Serving: Poisson arrivals swept over rates; each request is a seeded
+    Plummer system with N log-uniform in [1e3, 1e6] and a step count uniform in
+    [10, 1000], eps = 0.005, dt = 1e-4; direct method below N = 32,768,
+    Barnes-Hut (theta = 0.5) above; HBM budget 200 GB.  Requests are submitted
+    at their (replayed) arrival times while earlier ones run; reports p50/p99
+    completion latency (arrival -> result on the host) and aggregate
+    particle-steps/s per rate."""
+    import nbody_amd as nb
+
+    rates = [10, 25, 50, 100, 250, 500]
+    per_rate = args.requests
+    rows = []
+    for ri, rate in enumerate(rates):
+        rng = np.random.Generator(np.random.Philox(1000 + ri))
+        gaps = rng.exponential(1.0 / rate, per_rate)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        ns = np.rint(np.exp(rng.uniform(np.log(1e3), np.log(1e6), per_rate))).astype(int)
+        steps = rng.integers(10, 1001, per_rate)
+        systems = [nb.ic_plummer(int(n), 50000 + 100 * ri + k) for k, n in enumerate(ns)]
+        srv = nb.Server(hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=4)
+        t0 = time.perf_counter()
+        sub = 0
+        done = {}
+        rid_of = {}
+        while len(done) < per_rate:
+            now = time.perf_counter() - t0
+            while sub < per_rate and arrivals[sub] <= now:
+                rid_of[srv.submit(systems[sub], int(steps[sub]), softening=0.005, dt=1e-4)] = sub
+                sub += 1
+            st = srv.status()
+            if st["active"] == 0 and st["pending"] == 0:
+                if sub < per_rate:
+                    time.sleep(max(0.0, arrivals[sub] - (time.perf_counter() - t0)))
+                continue
+            srv.tick()
+            t = time.perf_counter() - t0
+            for rid in srv.poll():
+                k = rid_of[rid]
+                srv.result(rid)
+                done[k] = t - arrivals[k]
+        wall = time.perf_counter() - t0
+        lat = np.array([done[k] for k in range(per_rate)])
+        work = float(np.sum(ns * steps))
+        rows.append({"rate_per_s": rate, "requests": per_rate, "p50_s": float(np.percentile(lat, 50)),
+                     "p99_s": float(np.percentile(lat, 99)), "particle_steps_per_s": work / wall,
+                     "wall_s": wall, "bh_requests": int(np.sum(ns >= 32768))})
+        log(f"[serve] rate {rate}: p50 {rows[-1]['p50_s']:.2f} s, p99 {rows[-1]['p99_s']:.2f} s, "
+            f"{rows[-1]['particle_steps_per_s']:.3g} particle-steps/s")
+    best = max(r["particle_steps_per_s"] for r in rows)
+    return {
+        "metric": "interactions/s (direct) & particle-steps/s (BH, N=1M); serve p50/p99 latency",
+        "value": best, "unit": "particle-steps/s", "n_gpus": 1, "steps": per_rate, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded Poisson arrival traces, Plummer systems)",
+        "config": {"workload": f"serving: Poisson arrivals at {rates} req/s, {per_rate} requests "
+                               f"per rate, N log-uniform [1e3,1e6], steps U[10,1000], eps=0.005, "
+                               f"dt=1e-4, direct below 32768 else BH theta=0.5, HBM budget 200 GB"},
+        "sweep": rows,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,7 +268,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--workload", choices=("bh", "direct", "ensemble"), default="bh")
+    ap.add_argument("--workload", choices=("bh", "direct", "ensemble", "serve"), default="bh")
+    ap.add_argument("--requests", type=int, default=16)
     args = ap.parse_args()
     args.steps_set = args.steps is not None
     if args.gpus != 1:
@@ -213,7 +277,8 @@ def main():
                                    "engine has no multi-GPU path; run with --gpus 1"}), flush=True)
         sys.exit(2)
     if args.workload != "bh":
-        out = bench_direct(args) if args.workload == "direct" else bench_ensemble(args)
+        out = {"direct": bench_direct, "ensemble": bench_ensemble,
+               "serve": bench_serve}[args.workload](args)
         print(json.dumps(out), flush=True)
         return
 

@@ -264,6 +264,43 @@ int nb_ensemble_autotune(nb_ensemble* e, const char* path);
 int nb_ensemble_load_tuning(nb_ensemble* e, const char* path);
 int nb_ensemble_tuning(const nb_ensemble* e, int32_t* shape_codes, int n_buckets);
 
+/* ---------------------------------------------------------------- serving */
+/* Continuous-batching server: requests (one system each) are admitted against
+ * an HBM budget, keep their state resident on the device, and every tick
+ * advances every active request by one leapfrog step -- direct-method requests
+ * (n < direct_threshold) in one batched launch, Barnes-Hut requests on a pool
+ * of bh_lanes calculators with their own streams, overlapping on the device.
+ * Finished requests are downloaded and released; pending ones are admitted as
+ * memory frees up.  No reference counterpart (the reference runs one system
+ * per process). */
+typedef struct nb_server nb_server;
+typedef struct nb_request {
+    int64_t n;                  /* particles */
+    int32_t steps;              /* leapfrog steps to run */
+    double G, softening, dt;
+    double theta;               /* opening angle (Barnes-Hut requests) */
+    const double *m, *x, *y, *z, *vx, *vy, *vz; /* host arrays, copied at submission */
+} nb_request;
+typedef struct nb_server_status {
+    int64_t pending, active, finished_uncollected;
+    int64_t bytes_in_use;       /* budget charged to admitted requests and lanes */
+    int64_t ticks, particle_steps;
+} nb_server_status;
+int nb_server_create(nb_server** s, double hbm_budget_bytes, int64_t direct_threshold,
+                     int bh_lanes, int leaf_capacity);
+int nb_server_destroy(nb_server* s);
+/* Queues a request (NB_ERR_BUDGET if it can never fit). */
+int nb_server_submit(nb_server* s, const nb_request* r, int64_t* id);
+/* Admits what fits, advances every active request by one step, completes
+ * the finished ones.  *n_completed receives how many finished in this tick. */
+int nb_server_tick(nb_server* s, int* n_completed);
+/* Ids of requests finished since the last poll. */
+int nb_server_poll(nb_server* s, int64_t* ids, int capacity, int* count);
+/* Final positions/velocities of a finished request (then forgotten). */
+int nb_server_result(nb_server* s, int64_t id, double* x, double* y, double* z, double* vx,
+                     double* vy, double* vz);
+int nb_server_get_status(const nb_server* s, nb_server_status* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,7 +47,7 @@ constexpr int kSortRows = 16;
 constexpr int kSortTile = kBlk * kSortRows;
 constexpr int kSortBins = 256;
 constexpr int kSortPasses = 8;
-constexpr int kWalkFcap = 4096;  // walk frontier capacity per block (opened cells per level)
+constexpr int kWalkFcap = 2048;  // walk frontier capacity per block (opened cells per level)
 constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
 constexpr int kUpsweepGrid = 512;
 constexpr int kGroup = 64;      // walk group: cells of at most 64 particles

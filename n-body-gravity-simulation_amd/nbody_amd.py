@@ -63,6 +63,17 @@ class _Diag(C.Structure):
                 ("mass", C.c_double)]
 
 
+class _Request(C.Structure):
+    _fields_ = [("n", C.c_int64), ("steps", C.c_int32), ("G", C.c_double),
+                ("softening", C.c_double), ("dt", C.c_double), ("theta", C.c_double)] + \
+               [(k, C.c_void_p) for k in ("m", "x", "y", "z", "vx", "vy", "vz")]
+
+
+class _ServerStatus(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("pending", "active", "finished_uncollected",
+                                         "bytes_in_use", "ticks", "particle_steps")]
+
+
 _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _I64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _Dn = C.c_void_p  # nullable double*
@@ -120,6 +131,13 @@ def _load():
         "nb_ensemble_autotune": ([_vp, C.c_char_p], C.c_int),
         "nb_ensemble_load_tuning": ([_vp, C.c_char_p], C.c_int),
         "nb_ensemble_tuning": ([_vp, _vp, C.c_int], C.c_int),
+        "nb_server_create": ([C.POINTER(_vp), C.c_double, C.c_int64, C.c_int, C.c_int], C.c_int),
+        "nb_server_destroy": ([_vp], C.c_int),
+        "nb_server_submit": ([_vp, C.POINTER(_Request), C.POINTER(C.c_int64)], C.c_int),
+        "nb_server_tick": ([_vp, C.POINTER(C.c_int)], C.c_int),
+        "nb_server_poll": ([_vp, _vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
+        "nb_server_result": ([_vp, C.c_int64] + [_Dn] * 6, C.c_int),
+        "nb_server_get_status": ([_vp, C.POINTER(_ServerStatus)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -452,6 +470,59 @@ class Ensemble:
         try:
             if self._h:
                 lib.nb_ensemble_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------------------------- serving
+class Server:
+    """Continuous-batching server (see nb_server_* in include/nbody_amd.h)."""
+
+    def __init__(self, hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=4,
+                 leaf_capacity=8):
+        h = C.c_void_p()
+        _check(lib.nb_server_create(C.byref(h), float(hbm_budget_bytes), int(direct_threshold),
+                                    int(bh_lanes), int(leaf_capacity)))
+        self._h = h
+        self._n = {}
+
+    def submit(self, system, steps, G=1.0, softening=0.005, dt=1e-4, theta=0.5):
+        arr = [_f64(system[k]) for k in ("m", "x", "y", "z", "vx", "vy", "vz")]
+        r = _Request(len(arr[0]), int(steps), G, softening, dt, theta,
+                     *[a.ctypes.data for a in arr])
+        rid = C.c_int64(0)
+        _check(lib.nb_server_submit(self._h, C.byref(r), C.byref(rid)))
+        self._n[rid.value] = len(arr[0])
+        return rid.value
+
+    def tick(self):
+        c = C.c_int(0)
+        _check(lib.nb_server_tick(self._h, C.byref(c)))
+        return c.value
+
+    def poll(self, capacity=4096):
+        ids = np.zeros(capacity, np.int64)
+        c = C.c_int(0)
+        _check(lib.nb_server_poll(self._h, ids.ctypes.data, capacity, C.byref(c)))
+        return ids[:c.value].tolist()
+
+    def result(self, rid):
+        n = self._n.pop(rid)
+        out = {k: np.empty(n) for k in ("x", "y", "z", "vx", "vy", "vz")}
+        _check(lib.nb_server_result(self._h, rid, *[out[k].ctypes.data for k in
+                                                    ("x", "y", "z", "vx", "vy", "vz")]))
+        return out
+
+    def status(self):
+        st = _ServerStatus()
+        _check(lib.nb_server_get_status(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _ServerStatus._fields_}
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib.nb_server_destroy(self._h)
                 self._h = None
         except Exception:
             pass

@@ -1,0 +1,96 @@
+"""GPU tests of the continuous-batching server.
+
+Bars: every request's final state is bit-identical whether it runs alone or
+in a mix (direct requests share one batched launch, Barnes-Hut requests share
+lanes); a Barnes-Hut request equals the library's own BH + leapfrog path bit
+for bit; direct requests agree with the FP64 oracle to FP32 tolerance;
+admission respects the HBM budget."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+FIELDS = ("x", "y", "z", "vx", "vy", "vz")
+
+REQS = [  # (n, steps, seed)
+    (1000, 5, 1), (3000, 8, 2), (20000, 3, 3), (700, 6, 4), (40000, 4, 5), (50000, 6, 6),
+]
+
+
+def systems(nb):
+    return [nb.ic_plummer(n, seed) for n, _, seed in REQS]
+
+
+def run_server(nb, items, stagger=0, budget=200e9, lanes=2):
+    srv = nb.Server(hbm_budget_bytes=budget, direct_threshold=32768, bh_lanes=lanes)
+    ids = []
+    pending = list(items)
+    results = {}
+    ticks = 0
+    while pending or len(results) < len(items):
+        if pending and (stagger == 0 or ticks % stagger == 0):
+            sysd, steps = pending.pop(0)
+            ids.append(srv.submit(sysd, steps))
+            if stagger == 0:
+                continue
+        srv.tick()
+        ticks += 1
+        for rid in srv.poll():
+            results[rid] = srv.result(rid)
+        assert ticks < 10000
+    return [results[i] for i in ids], srv
+
+
+def test_same_alone_and_in_the_mix(nb):
+    sy = systems(nb)
+    items = [(s, steps) for s, (_, steps, _) in zip(sy, REQS)]
+    mix, _ = run_server(nb, items)
+    staggered, _ = run_server(nb, items, stagger=2)
+    for k, it in enumerate(items):
+        alone, _ = run_server(nb, [it])
+        for f in FIELDS:
+            assert np.array_equal(alone[0][f], mix[k][f]), (k, f)
+            assert np.array_equal(alone[0][f], staggered[k][f]), (k, f)
+
+
+def test_bh_request_equals_library_path(nb):
+    n, steps, seed = REQS[4]
+    s = nb.ic_plummer(n, seed)
+    res, _ = run_server(nb, [(s, steps)])
+    f = nb.BarnesHutForceCalculator(0.005, 0.5, 8, "quadrupole", G=1.0)
+    p = nb.Particles.from_arrays(s["m"], s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+    it = nb.LeapfrogIntegrator()
+    for _ in range(steps):
+        it.step(f, p, 1e-4)
+    g = p.download()
+    for fl in FIELDS:
+        assert np.array_equal(g[fl], res[0][fl]), fl
+
+
+def test_direct_request_vs_oracle(nb, O):
+    n, steps, seed = REQS[1]
+    s = nb.ic_plummer(n, seed)
+    res, _ = run_server(nb, [(s, steps)])
+    ref = O.leapfrog_direct(s["m"], s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"], 1e-4, steps,
+                            1.0, 0.005 ** 2, False)
+    # FP32 forces (relative error ~1e-6 of |a| <~ 100 here) integrated over
+    # steps*dt = 8e-4 bound the velocity error by ~1e-7 and positions by ~1e-10.
+    for fl, r in zip(FIELDS, ref[:6]):
+        tol = 1e-10 if fl in ("x", "y", "z") else 1e-7
+        assert np.abs(res[0][fl] - r).max() < tol, fl
+
+
+def test_budget_admission(nb):
+    sy = systems(nb)
+    items = [(sy[0], 3), (sy[1], 3), (sy[3], 3)]
+    # Room for one direct request at a time: the others wait, then run.
+    budget = 3000 * 96 * 1.5
+    res, srv = run_server(nb, items, budget=budget)
+    alone = [run_server(nb, [it])[0][0] for it in items]
+    for a, b in zip(alone, res):
+        assert all(np.array_equal(a[f], b[f]) for f in FIELDS)
+    st = srv.status()
+    assert st["active"] == 0 and st["pending"] == 0 and st["bytes_in_use"] == 0
+    too_big = nb.Server(hbm_budget_bytes=1e5)
+    with pytest.raises(nb.NBError) as e:
+        too_big.submit(sy[2], 1)
+    assert e.value.code == 5  # NB_ERR_BUDGET
