@@ -258,6 +258,24 @@ def bench_serve(args):
     }
 
 
+def copy_probe(nbytes=1 << 28):
+    """A plain device-to-device copy of `nbytes` (HIP runtime copy kernel),
+    used to calibrate the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,8 +288,13 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workload", choices=("bh", "direct", "ensemble", "serve"), default="bh")
     ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
     args.steps_set = args.steps is not None
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: every box has one MI355X and the "
                                    "engine has no multi-GPU path; run with --gpus 1"}), flush=True)
