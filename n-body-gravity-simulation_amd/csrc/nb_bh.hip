@@ -92,6 +92,7 @@ struct nb::BHWorkspace {
     DevBuf n_pstart, n_pend, n_child, n_cend, n_level, n_mom, w_mac, w_link, w_q0, w_q1;
     DevBuf walk_scratch, gflag, gstart, glev, totals;
     int walk_blocks = 0;
+    int group_size = 0;
 };
 
 namespace nb {
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(kBlk) void k_gather(int n, const unsigned* __restri
 // kGroup particles (or the particle's leaf if that is shallower), so a group
 // is spatially compact; its particles are contiguous in sorted order.
 __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long* __restrict__ keys,
-                                                 int cap, const RootInfo* __restrict__ root,
+                                                 int cap, int gsz, const RootInfo* __restrict__ root,
                                                  signed char* __restrict__ Lout,
                                                  signed char* __restrict__ shout,
                                                  unsigned char* __restrict__ gflag,
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
     }
     __syncthreads();
     // ss[u]: shared levels of keys (j, j+cap) with j = b0 - cap + u;
-    // sg[u]: shared levels of keys (j, j+kGroup) with j = b0 - kGroup + u.
+    // sg[u]: shared levels of keys (j, j+gsz) with j = b0 - gsz + u (gsz <= kGroup).
     for (int u = t; u < kBlk + H; u += kBlk) {
         if (u < kBlk + cap) {
             const int j = b0 - cap + u;
@@ -410,9 +411,12 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
                         ? (signed char)shared_levels(sk[H - cap + u], sk[H + u])
                         : (signed char)-1;
         }
-        const int j = b0 - kGroup + u;
-        sg[u] = (j >= 0 && j + kGroup < n) ? (signed char)shared_levels(sk[u], sk[kGroup + u])
-                                           : (signed char)-1;
+        if (u < kBlk + gsz) {
+            const int jg = b0 - gsz + u;
+            sg[u] = (jg >= 0 && jg + gsz < n)
+                        ? (signed char)shared_levels(sk[H - gsz + u], sk[H + u])
+                        : (signed char)-1;
+        }
     }
     __syncthreads();
     const int i = b0 + t;
@@ -422,7 +426,7 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
     if (valid) {
         for (int q = 0; q <= cap; ++q) mx = max(mx, (int)ss[t + q]);
         int mg = -1;
-        for (int q = 0; q <= kGroup; ++q) mg = max(mg, (int)sg[t + q]);
+        for (int q = 0; q <= gsz; ++q) mg = max(mg, (int)sg[t + q]);
         Li = min(lmax, mx + 1);
         gl = min(Li, mg + 1);
         shi = (i == 0) ? -1 : shared_levels(sk[H + t - 1], sk[H + t]);
@@ -1044,7 +1048,12 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlk), 0, s, n, v0, x, y, z, m,
                        w->spos.as<float4>());
     if (ev) NB_HIP(hipEventRecord(ev[4], s));
-    hipLaunchKernelGGL(k_levels, dim3(nblk), dim3(kBlk), 0, s, n, k0, leaf_cap, root,
+    if (w->group_size == 0) {  // walk group size: NB_WALK_GROUP overrides the default
+        w->group_size = kGroup;
+        if (const char* e = std::getenv("NB_WALK_GROUP"))
+            w->group_size = std::max(1, std::min(kGroup, std::atoi(e)));
+    }
+    hipLaunchKernelGGL(k_levels, dim3(nblk), dim3(kBlk), 0, s, n, k0, leaf_cap, w->group_size, root,
                        w->lvl_leaf.as<signed char>(), w->lvl_sh.as<signed char>(),
                        w->gflag.as<unsigned char>(), w->glev.as<signed char>(), nblk,
                        w->blkcnt.as<int>(), ctr);
