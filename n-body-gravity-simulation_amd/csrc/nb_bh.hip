@@ -22,15 +22,16 @@
 //   up-sweep  k_upsweep per level (deepest first): mass, centre of mass and the
 //             traceless quadrupole in FP64 for every level; FP32 copies for
 //             the walk.
-//   walk      k_walk: one wave64 per group of 64 consecutive sorted targets.
-//             The wave opens cells breadth-first (8 parents x 8 children per
-//             step, one child per lane), accepting a cell for the whole group
-//             when size^2 < theta^2 * d^2 with d the distance from the cell's
-//             centre of mass to the group's bounding box (the reference
-//             criterion of force_calculator_barnes_hut_optimized.cpp:88,
-//             evaluated conservatively for all 64 targets at once).  Accepted
-//             cells and opened leaves' particles are staged in LDS and
-//             evaluated 64 at a time by all lanes (one target per lane).
+//   walk      k_walk: persistent one-wave blocks take walk groups (the tree
+//             cells of at most 64 particles) from an atomic counter.  The wave
+//             opens cells breadth-first (8 parents x 8 children per step, one
+//             child per lane), accepting a cell for the whole group when
+//             size^2 < theta^2 * d^2 with d the distance from the cell's centre
+//             of mass to the group's bounding box (the reference criterion of
+//             force_calculator_barnes_hut_optimized.cpp:88, evaluated
+//             conservatively for every target of the group at once).  Accepted
+//             cells and opened leaves' particles are staged in LDS 64 at a time
+//             and evaluated by all lanes, two targets per lane.
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -698,7 +699,7 @@ __global__ __launch_bounds__(kBlk) void k_upsweep(
 // opened cells and lives in a per-block slice of global scratch (it can reach
 // thousands of entries in dense regions); interaction lists live in LDS.
 template <bool kQuad, bool kEps0>
-__global__ __launch_bounds__(64) void k_walk(int n, const float4* __restrict__ spos,
+__global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict__ spos,
                                              const unsigned* __restrict__ perm,
                                              const float4* __restrict__ mac,
                                              const int2* __restrict__ link,
@@ -735,21 +736,26 @@ __global__ __launch_bounds__(64) void k_walk(int n, const float4* __restrict__ s
         __syncthreads();
         if (grp >= ngroups) break;
         const int gend = gstart[grp + 1];
-      for (int g0 = gstart[grp]; g0 < gend; g0 += 64) {  // groups above 64 (oversize leaves) in chunks
-        const int nact = min(64, gend - g0);
-        // A group of nact <= 32 targets runs on nsub = 64/nact lane subsets; each
-        // subset takes every nsub-th entry of the interaction lists and the
-        // partial sums are added in subset order at the end.
-        const int nsub = 64 / nact;
-        const int sub = lane / nact;
-        const int tix = lane - sub * nact;
+      for (int g0 = gstart[grp]; g0 < gend; g0 += 128) {  // oversize leaves in chunks of 128
+        const int nact = min(128, gend - g0);
+        // Every lane holds two targets, A = tix and B = tix + nl, of a subset of
+        // nl = ceil(nact/2) lanes.  The wave runs nsub = 64/nl such subsets; each
+        // takes every nsub-th entry of the interaction lists and the partial
+        // sums are added in subset order at the end.
+        const int nl = (nact + 1) >> 1;
+        const int nsub = 64 / nl;
+        const int sub = lane / nl;
+        const int tix = lane - sub * nl;
         const bool live = sub < nsub;
-        const int k = g0 + tix;
-        const bool active = live && sub == 0;
-        const float4 me = spos[live ? k : g0];
+        const bool hasB = live && tix + nl < nact;
+        const bool lead = live && sub == 0;
+        const int kA = g0 + tix, kB = g0 + tix + nl;
+        const float4 meA = spos[live ? kA : g0];
+        const float4 meB = hasB ? spos[kB] : meA;  // a lane without B computes A twice
 
         // Group bounding box.
-        float lo0 = me.x, lo1 = me.y, lo2 = me.z, hi0 = me.x, hi1 = me.y, hi2 = me.z;
+        float lo0 = fminf(meA.x, meB.x), lo1 = fminf(meA.y, meB.y), lo2 = fminf(meA.z, meB.z);
+        float hi0 = fmaxf(meA.x, meB.x), hi1 = fmaxf(meA.y, meB.y), hi2 = fmaxf(meA.z, meB.z);
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             lo0 = fminf(lo0, __shfl_xor(lo0, d, 64));
@@ -763,24 +769,57 @@ __global__ __launch_bounds__(64) void k_walk(int n, const float4* __restrict__ s
         const float bh0 = fmaxf(hi0 - bc0, bc0 - lo0), bh1 = fmaxf(hi1 - bc1, bc1 - lo1),
                     bh2 = fmaxf(hi2 - bc2, bc2 - lo2);
 
-        float fx = 0.f, fy = 0.f, fz = 0.f;
+        float fxA = 0.f, fyA = 0.f, fzA = 0.f, fxB = 0.f, fyB = 0.f, fzB = 0.f;
         int ncl = 0, npl = 0, nnext = 0;
         unsigned long long npp = 0, npc = 0;
         int2* nxt = fr0;
 
+        auto pp_one = [&](const float4& s, const float4& me, float& fx, float& fy, float& fz) {
+            const float dx = s.x - me.x, dy = s.y - me.y, dz = s.z - me.z;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
+            float inv = __builtin_amdgcn_rsqf(r2);
+            if (kEps0) inv = r2 > 0.f ? inv : 0.f;
+            const float c = (s.w * inv) * (inv * inv);
+            fx = fmaf(dx, c, fx);
+            fy = fmaf(dy, c, fy);
+            fz = fmaf(dz, c, fz);
+        };
+
         auto flush_particles = [&](int cnt) {
             for (int j = live ? sub : cnt; j < cnt; j += nsub) {
                 const float4 s = pl[j];
-                const float dx = s.x - me.x, dy = s.y - me.y, dz = s.z - me.z;
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
-                float inv = __builtin_amdgcn_rsqf(r2);
-                if (kEps0) inv = r2 > 0.f ? inv : 0.f;
-                const float c = (s.w * inv) * (inv * inv);
-                fx = fmaf(dx, c, fx);
-                fy = fmaf(dy, c, fy);
-                fz = fmaf(dz, c, fz);
+                pp_one(s, meA, fxA, fyA, fzA);
+                pp_one(s, meB, fxB, fyB, fzB);
             }
             npp += (unsigned long long)cnt;
+        };
+
+        // Quadrupole term: Q = [[xx xy xz],[xy yy yz],[xz yz zz]] with
+        // qa = (M, xx, yy, zz), qb = (xy, xz, yz).
+        auto pc_one = [&](const float4& c, const float4& qa, const float4& qb, const float4& me,
+                          float& fx, float& fy, float& fz) {
+            const float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float inv2 = inv * inv;
+            const float inv3 = inv * inv2;
+            if (kQuad) {
+                const float qdx = fmaf(qa.y, dx, fmaf(qb.x, dy, qb.y * dz));
+                const float qdy = fmaf(qb.x, dx, fmaf(qa.z, dy, qb.z * dz));
+                const float qdz = fmaf(qb.y, dx, fmaf(qb.z, dy, qa.w * dz));
+                const float dqd = fmaf(dx, qdx, fmaf(dy, qdy, dz * qdz));
+                const float inv5 = inv3 * inv2;
+                const float inv7 = inv5 * inv2;
+                const float cm = fmaf(2.5f * dqd, inv7, qa.x * inv3);
+                fx = fmaf(dx, cm, fmaf(-qdx, inv5, fx));
+                fy = fmaf(dy, cm, fmaf(-qdy, inv5, fy));
+                fz = fmaf(dz, cm, fmaf(-qdz, inv5, fz));
+            } else {
+                const float cm = qa.x * inv3;
+                fx = fmaf(dx, cm, fx);
+                fy = fmaf(dy, cm, fy);
+                fz = fmaf(dz, cm, fz);
+            }
         };
 
         auto flush_cells = [&](int cnt) {
@@ -794,30 +833,9 @@ __global__ __launch_bounds__(64) void k_walk(int n, const float4* __restrict__ s
             for (int j = live ? sub : cnt; j < cnt; j += nsub) {
                 const float4 c = st0[j];
                 const float4 qa = st1[j];
-                const float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
-                const float inv = __builtin_amdgcn_rsqf(r2);
-                const float inv2 = inv * inv;
-                const float inv3 = inv * inv2;
-                if (kQuad) {
-                    const float4 qb = st2[j];
-                    // Q = [[xx xy xz],[xy yy yz],[xz yz zz]]: qa = (M, xx, yy, zz), qb = (xy, xz, yz)
-                    const float qdx = fmaf(qa.y, dx, fmaf(qb.x, dy, qb.y * dz));
-                    const float qdy = fmaf(qb.x, dx, fmaf(qa.z, dy, qb.z * dz));
-                    const float qdz = fmaf(qb.y, dx, fmaf(qb.z, dy, qa.w * dz));
-                    const float dqd = fmaf(dx, qdx, fmaf(dy, qdy, dz * qdz));
-                    const float inv5 = inv3 * inv2;
-                    const float inv7 = inv5 * inv2;
-                    const float cm = fmaf(2.5f * dqd, inv7, qa.x * inv3);
-                    fx = fmaf(dx, cm, fmaf(-qdx, inv5, fx));
-                    fy = fmaf(dy, cm, fmaf(-qdy, inv5, fy));
-                    fz = fmaf(dz, cm, fmaf(-qdz, inv5, fz));
-                } else {
-                    const float cm = qa.x * inv3;
-                    fx = fmaf(dx, cm, fx);
-                    fy = fmaf(dy, cm, fy);
-                    fz = fmaf(dz, cm, fz);
-                }
+                const float4 qb = kQuad ? st2[j] : qa;
+                pc_one(c, qa, qb, meA, fxA, fyA, fzA);
+                pc_one(c, qa, qb, meB, fxB, fyB, fzB);
             }
             npc += (unsigned long long)cnt;
             __syncthreads();
@@ -925,23 +943,34 @@ __global__ __launch_bounds__(64) void k_walk(int n, const float4* __restrict__ s
         if (npl > 0) flush_particles(npl);
 
         if (nsub > 1) {  // reduce the subsets' partial sums in subset order
-            st0[lane] = make_float4(fx, fy, fz, 0.f);
+            st0[lane] = make_float4(fxA, fyA, fzA, 0.f);
+            st1[lane] = make_float4(fxB, fyB, fzB, 0.f);
             __syncthreads();
-            if (active) {
+            if (lead) {
                 for (int q = 1; q < nsub; ++q) {
-                    const float4 v = st0[q * nact + tix];
-                    fx += v.x;
-                    fy += v.y;
-                    fz += v.z;
+                    const float4 a = st0[q * nl + tix];
+                    const float4 b = st1[q * nl + tix];
+                    fxA += a.x;
+                    fyA += a.y;
+                    fzA += a.z;
+                    fxB += b.x;
+                    fyB += b.y;
+                    fzB += b.z;
                 }
             }
             __syncthreads();
         }
-        if (active) {
-            const unsigned j = perm[k];
-            ax[j] = G * (double)fx;
-            ay[j] = G * (double)fy;
-            az[j] = G * (double)fz;
+        if (lead) {
+            const unsigned j = perm[kA];
+            ax[j] = G * (double)fxA;
+            ay[j] = G * (double)fyA;
+            az[j] = G * (double)fzA;
+            if (hasB) {
+                const unsigned jb = perm[kB];
+                ax[jb] = G * (double)fxB;
+                ay[jb] = G * (double)fyB;
+                az[jb] = G * (double)fzB;
+            }
         }
         tot_pp += npp * (unsigned long long)nact;
         tot_pc += npc * (unsigned long long)nact;
