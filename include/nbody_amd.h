@@ -121,10 +121,14 @@ const char* nb_force_name(const nb_force* f);
  * CPU/include/force_calculator.hpp:30; CUDA BarnesHutForceCalculator getters,
  * CUDA/include/force_calculator_barnes_hut.hpp:120-125).  Recording adds no
  * synchronisation; reading the timings synchronizes the stream.
+ * nb_force_set_timing: 0 = off, 1 = every Barnes-Hut stage, 2 = only tree
+ * build and walk (fewer events: each recorded event leaves the GPU idle for a
+ * few microseconds between kernels).
  * nb_force_timing: out[0]=total, out[1]=tree build (bbox..up-sweep),
  * out[2]=walk (the whole direct sum for direct calculators), out[3]=calls.
  * nb_force_stage_timing: out[0..6] = bbox, keys, sort, gather, build,
- * up-sweep, walk; out[7] = calls (if n_out > 7). */
+ * up-sweep, walk; out[7] = calls (if n_out > 7).  Calls timed in mode 2
+ * contribute to the walk entry only. */
 int nb_force_set_timing(nb_force* f, int enabled);
 int nb_force_timing(nb_force* f, double out[4]);
 int nb_force_stage_timing(nb_force* f, double* out, int n_out);

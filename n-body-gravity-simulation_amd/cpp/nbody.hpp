@@ -220,7 +220,7 @@ class BarnesHutForceCalculator : public ForceCalculator {
         G_ = G;
         nbody::check(nb_force_create_bh(&h_, G, softening, theta, leaf_capacity, multipole),
                      "BarnesHutForceCalculator");
-        nb_force_set_timing(h_, 1);
+        nb_force_set_timing(h_, 2);  // build + walk: the getters below need no more
         stats_.name = "Barnes-Hut Force Calculation";
     }
     nb_bh_stats stats() const {

@@ -1056,10 +1056,10 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->bbox_part.as<double>());
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlk), 0, s, nbb, w->bbox_part.as<double>(),
                        root);
-    if (ev) NB_HIP(hipEventRecord(ev[1], s));
+    if (ev && ev[1]) NB_HIP(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(k_keys, dim3(nblk), dim3(kBlk), 0, s, n, x, y, z, root, k0, v0);
     NB_LAUNCH_CHECK();
-    if (ev) NB_HIP(hipEventRecord(ev[2], s));
+    if (ev && ev[2]) NB_HIP(hipEventRecord(ev[2], s));
     for (int p = 0; p < kSortPasses; ++p) {
         const int shift = 8 * p;
         hipLaunchKernelGGL(k_radix_hist, dim3(nsort), dim3(kBlk), 0, s, n, k0, shift, nsort,
@@ -1073,10 +1073,10 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     }
     NB_LAUNCH_CHECK();
     // kSortPasses is even: the sorted keys are back in keys0/vals0.
-    if (ev) NB_HIP(hipEventRecord(ev[3], s));
+    if (ev && ev[3]) NB_HIP(hipEventRecord(ev[3], s));
     hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlk), 0, s, n, v0, x, y, z, m,
                        w->spos.as<float4>());
-    if (ev) NB_HIP(hipEventRecord(ev[4], s));
+    if (ev && ev[4]) NB_HIP(hipEventRecord(ev[4], s));
     if (w->group_size == 0) {  // walk group size: NB_WALK_GROUP overrides the default
         w->group_size = kGroup;
         if (const char* e = std::getenv("NB_WALK_GROUP"))
@@ -1096,7 +1096,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->n_cend.as<int>(), w->n_level.as<signed char>(), w->leaf_of.as<int>(),
                        w->gflag.as<unsigned char>(), w->gstart.as<int>(), ctr);
     NB_LAUNCH_CHECK();
-    if (ev) NB_HIP(hipEventRecord(ev[5], s));
+    if (ev && ev[5]) NB_HIP(hipEventRecord(ev[5], s));
     const double inv_theta2 = theta > 0.0 ? 1.0 / (theta * theta) : INFINITY;
     for (int l = kLevels - 1; l >= 0; --l) {
         hipLaunchKernelGGL(k_upsweep, dim3(kUpsweepGrid), dim3(kBlk), 0, s, l, li, root,
@@ -1107,7 +1107,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                            inv_theta2, multipole == NB_QUADRUPOLE ? 1 : 0, n, ctr);
     }
     NB_LAUNCH_CHECK();
-    if (ev) NB_HIP(hipEventRecord(ev[6], s));
+    if (ev && ev[6]) NB_HIP(hipEventRecord(ev[6], s));
 
     const int ngroups = grid_for(n, 16);  // upper bound on walk groups in flight
     if (w->walk_blocks == 0) {
@@ -1136,7 +1136,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     }
 #undef NB_WALK
     NB_LAUNCH_CHECK();
-    if (ev) NB_HIP(hipEventRecord(ev[7], s));
+    if (ev && ev[7]) NB_HIP(hipEventRecord(ev[7], s));
     w->last_n = n;
     w->last_cap = leaf_cap;
     w->last_valid = 1;

@@ -95,14 +95,18 @@ struct nb_force {
     nb::DirectWS direct;
     nb::BHWorkspace* bh = nullptr;
     std::string name;
-    // Timing: when enabled, every call records kStages+1 HIP events taken from
-    // a pool on the calculator's stream (no synchronisation); the elapsed times
-    // are resolved when the timings are read.
-    bool timing = false;
+    // Timing (no synchronisation; resolved when the timings are read): every
+    // call records HIP events taken from a pool on the calculator's stream.
+    // timing == 1: one event per Barnes-Hut stage boundary; timing == 2: only
+    // the start, the start of the walk and the end (each recorded event costs
+    // a few microseconds of idle GPU between kernels).  A direct sum records
+    // only the start and end of its one stage.
+    int timing = 0;
     std::vector<hipEvent_t> ev_pool;
-    std::vector<int> ev_pending;  // first pool index of each unresolved call
+    std::vector<std::pair<int, int>> ev_pending;  // (first pool index, events recorded) per call
     size_t ev_used = 0;
     double t_stage[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double t_build_coarse = 0;  // tree-build time of timing == 2 calls
     int64_t timed_calls = 0;
     int64_t calls = 0;
 };

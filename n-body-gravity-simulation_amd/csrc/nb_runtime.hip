@@ -40,14 +40,27 @@ void DevBuf::release() {
     bytes = 0;
 }
 
+namespace {
+// Which events a timed call records.
+enum { kEvAll = 0, kEvCoarse = 1, kEvWalkOnly = 2 };
+}  // namespace
+
 int resolve_timing(nb_force* f) {
     if (f->ev_pending.empty()) return NB_OK;
     NB_HIP(hipStreamSynchronize(f->stream));
-    for (int b : f->ev_pending) {
-        for (int k = 0; k < kStages; ++k) {
+    for (const auto& pe : f->ev_pending) {
+        const hipEvent_t* e = &f->ev_pool[pe.first];
+        auto elapsed = [&](int a, int b, double& acc) -> int {
             float ms = 0.f;
-            NB_HIP(hipEventElapsedTime(&ms, f->ev_pool[b + k], f->ev_pool[b + k + 1]));
-            f->t_stage[k] += ms;
+            NB_HIP(hipEventElapsedTime(&ms, e[a], e[b]));
+            acc += ms;
+            return NB_OK;
+        };
+        if (pe.second == kEvAll) {
+            for (int k = 0; k < kStages; ++k) NB_TRY(elapsed(k, k + 1, f->t_stage[k]));
+        } else {
+            if (pe.second == kEvCoarse) NB_TRY(elapsed(0, kStages - 1, f->t_build_coarse));
+            NB_TRY(elapsed(kStages - 1, kStages, f->t_stage[kStages - 1]));
         }
     }
     f->timed_calls += (int64_t)f->ev_pending.size();
@@ -59,6 +72,7 @@ int resolve_timing(nb_force* f) {
 int force_eval(nb_force* f, int64_t n, const double* m, const double* x, const double* y,
                const double* z, double* ax, double* ay, double* az) {
     if (n <= 0) return NB_OK;
+    hipEvent_t evs[kStages + 1] = {};  // null entries are not recorded
     const hipEvent_t* ev = nullptr;
     if (f->timing) {
         if (f->ev_pending.size() >= 256) NB_TRY(resolve_timing(f));  // bound the event pool
@@ -68,13 +82,15 @@ int force_eval(nb_force* f, int64_t n, const double* m, const double* x, const d
             NB_HIP(hipEventCreate(&e));
             f->ev_pool.push_back(e);
         }
-        f->ev_pending.push_back((int)f->ev_used);
-        ev = &f->ev_pool[f->ev_used];
+        const int mode = f->kind == 0 ? kEvWalkOnly : f->timing == 1 ? kEvAll : kEvCoarse;
+        f->ev_pending.emplace_back((int)f->ev_used, mode);
+        const hipEvent_t* base = &f->ev_pool[f->ev_used];
         f->ev_used = need;
-        // A direct sum has no tree stages: ev[0..kStages-1] are recorded back to
-        // back before it, so its whole time is booked as the last ("walk") stage.
-        const int first = f->kind == 0 ? kStages - 1 : 0;
-        for (int k = 0; k <= first; ++k) NB_HIP(hipEventRecord(ev[k], f->stream));
+        for (int k = 0; k <= kStages; ++k)
+            if (mode == kEvAll || k >= kStages - 1 || (mode == kEvCoarse && k == 0)) evs[k] = base[k];
+        ev = evs;
+        // A direct sum has no tree stages: its whole time is the last ("walk") stage.
+        NB_HIP(hipEventRecord(f->kind == 0 ? ev[kStages - 1] : ev[0], f->stream));
     }
     if (f->kind == 0) {
         NB_TRY(direct_eval(f->precision, f->G, f->soft2, n, m, x, y, z, ax, ay, az, f->direct,
@@ -270,19 +286,21 @@ int nb_force_compute(nb_force* f, nb_particles* p) {
 const char* nb_force_name(const nb_force* f) { return f ? f->name.c_str() : ""; }
 
 int nb_force_set_timing(nb_force* f, int enabled) {
-    if (!f) return NB_ERR_INVALID;
+    if (!f || enabled < 0 || enabled > 2) {
+        set_error("nb_force_set_timing: mode must be 0, 1 or 2");
+        return NB_ERR_INVALID;
+    }
     if (!enabled) NB_TRY(resolve_timing(f));
-    f->timing = enabled != 0;
+    f->timing = enabled;
     return NB_OK;
 }
 
 int nb_force_timing(nb_force* f, double out[4]) {
     if (!f || !out) return NB_ERR_INVALID;
     NB_TRY(resolve_timing(f));
-    double total = 0, build = 0;
-    for (int k = 0; k < kStages; ++k) total += f->t_stage[k];
+    double build = f->t_build_coarse;
     for (int k = 0; k < kStages - 1; ++k) build += f->t_stage[k];
-    out[0] = total;
+    out[0] = build + f->t_stage[kStages - 1];
     out[1] = build;
     out[2] = f->t_stage[kStages - 1];
     out[3] = (double)f->timed_calls;
@@ -301,6 +319,7 @@ int nb_force_reset_timing(nb_force* f) {
     if (!f) return NB_ERR_INVALID;
     NB_TRY(resolve_timing(f));
     for (double& t : f->t_stage) t = 0;
+    f->t_build_coarse = 0;
     f->timed_calls = 0;
     return NB_OK;
 }

@@ -235,8 +235,10 @@ class ForceCalculator:
     def name(self) -> str:
         return lib.nb_force_name(self._h).decode()
 
-    def set_timing(self, on=True):
-        _check(lib.nb_force_set_timing(self._h, int(on)))
+    def set_timing(self, on=True, stages=True):
+        """on=False: off; stages=True: every Barnes-Hut stage; stages=False:
+        tree build and walk only (fewer events on the stream)."""
+        _check(lib.nb_force_set_timing(self._h, 0 if not on else 1 if stages else 2))
 
     def timing(self):
         out = np.zeros(4)
