@@ -177,7 +177,17 @@ enum nb_integrator_kind {
      * CPU form CPU/src/integrator_leapfrog.cpp:3-26 for deterministic forces. */
     NB_LEAPFROG = 0,
     /* Classic RK4 (CPU/src/integrator_rk4.cpp:3-47). */
-    NB_RK4 = 1
+    NB_RK4 = 1,
+    /* Explicit Euler (CPU/src/integrator_euler.cpp:3-20, CUDA/src/integrator_euler.cu:24-50). */
+    NB_EULER = 2,
+    /* Euler-Cromer (CPU/src/integrator_euler-cromer.cpp:3-20,
+     * CUDA/src/integrator_euler-cromer.cu). */
+    NB_EULER_CROMER = 3,
+    /* Velocity Verlet with the acceleration cached between steps, as
+     * CUDA/src/integrator_velocity_verlet.cu:46-102 does; bit-identical to the
+     * two-eval CPU form CPU/src/integrator_velocity_verlet.cpp:3-23 for
+     * deterministic forces. */
+    NB_VELOCITY_VERLET = 4
 };
 
 int nb_integrator_create(nb_integrator** it, int kind);

@@ -21,6 +21,9 @@
 #pragma once
 
 #include <cmath>
+#include <iomanip>
+#include <iostream>
+#include <ostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -38,9 +41,16 @@ struct Vector3D {
     Vector3D() : x(0), y(0), z(0) {}
     Vector3D(double x_, double y_, double z_) : x(x_), y(y_), z(z_) {}
     double norm() const { return std::sqrt(x * x + y * y + z * z); }
+    Vector3D normalize() const {
+        const double n = norm();
+        if (n == 0) return Vector3D(0, 0, 0);
+        return Vector3D(x / n, y / n, z / n);
+    }
     Vector3D operator+(const Vector3D& o) const { return {x + o.x, y + o.y, z + o.z}; }
     Vector3D operator-(const Vector3D& o) const { return {x - o.x, y - o.y, z - o.z}; }
     Vector3D operator*(double s) const { return {x * s, y * s, z * s}; }
+    friend Vector3D operator*(double s, const Vector3D& v) { return {v.x * s, v.y * s, v.z * s}; }
+    Vector3D operator/(double s) const { return {x / s, y / s, z / s}; }
     Vector3D& operator+=(const Vector3D& o) {
         x += o.x;
         y += o.y;
@@ -48,6 +58,9 @@ struct Vector3D {
         return *this;
     }
     double dot(const Vector3D& o) const { return x * o.x + y * o.y + z * o.z; }
+    Vector3D cross(const Vector3D& o) const {
+        return {y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x};
+    }
 };
 
 struct Particle {
@@ -57,6 +70,19 @@ struct Particle {
     Particle() : mass(1) {}
     Particle(double m, const Vector3D& p, const Vector3D& v, const std::string& n = "")
         : mass(m), position(p), velocity(v), name(n) {}
+    Particle(double m, const Vector3D& p, const Vector3D& v, const Vector3D& a,
+             const std::string& n = "")
+        : mass(m), position(p), velocity(v), acceleration(a), name(n) {}
+    void info() const {
+        if (name.size()) std::cout << "Particle: " << name << "\n";
+        std::cout << "Mass: " << mass << "\n\n";
+        std::cout << "Position:     (" << position.x << ", " << position.y << ", " << position.z
+                  << ")\n";
+        std::cout << "Velocity:     (" << velocity.x << ", " << velocity.y << ", " << velocity.z
+                  << ")\n";
+        std::cout << "Acceleration: (" << acceleration.x << ", " << acceleration.y << ", "
+                  << acceleration.z << ")\n";
+    }
 };
 
 // CPU/include/common.hpp:40-51
@@ -298,3 +324,109 @@ class RK4Integrator : public TimeIntegrator {
    public:
     RK4Integrator() : TimeIntegrator(NB_RK4) {}
 };
+
+// CPU/include/integrator_euler.hpp, CUDA/include/integrator_euler.hpp
+class EulerIntegrator : public TimeIntegrator {
+   public:
+    EulerIntegrator() : TimeIntegrator(NB_EULER) {}
+};
+
+// CPU/include/integrator_euler-cromer.hpp, CUDA/include/integrator_euler-cromer.hpp
+class EulerCromerIntegrator : public TimeIntegrator {
+   public:
+    EulerCromerIntegrator() : TimeIntegrator(NB_EULER_CROMER) {}
+};
+
+// CPU/include/integrator_velocity_verlet.hpp, CUDA/include/integrator_velocity_verlet.hpp
+class VelocityVerletIntegrator : public TimeIntegrator {
+   public:
+    VelocityVerletIntegrator() : TimeIntegrator(NB_VELOCITY_VERLET) {}
+};
+
+// ------------------------------------------------------------------ test helpers
+// runSimulation and the diagnostics of CPU/include/tests_common.hpp:19-101,
+// with the same names and meaning.  The sums run on the GPU in FP64 with
+// compensated (Neumaier) accumulation; the potential is the reference's
+// unsoftened pair sum with G = constant::G.
+template <typename Integrator, typename Calculator>
+std::vector<Particle> runSimulation(std::vector<Particle> particles, Integrator& integrator,
+                                    Calculator& calculator, double dt, int steps) {
+    for (int i = 0; i < steps; i++) integrator.step(particles, calculator, dt);
+    return particles;
+}
+
+namespace nbody {
+inline nb_diag diagnostics(const std::vector<Particle>& particles, bool with_potential,
+                           double G = constant::G) {
+    DeviceMirror mirror;
+    mirror.upload(particles);
+    nb_diag d{};
+    check(nb_diagnostics(&mirror.d, G, 0.0, with_potential ? 1 : 0, &d), "diagnostics");
+    return d;
+}
+}  // namespace nbody
+
+inline double kineticEnergy(const std::vector<Particle>& particles) {
+    return nbody::diagnostics(particles, false).ke;
+}
+inline double potentialEnergy(const std::vector<Particle>& particles) {
+    return nbody::diagnostics(particles, true).pe;
+}
+inline double totalEnergy(const std::vector<Particle>& particles) {
+    const nb_diag d = nbody::diagnostics(particles, true);
+    return d.ke + d.pe;
+}
+inline Vector3D angularMomentum(const std::vector<Particle>& particles) {
+    const nb_diag d = nbody::diagnostics(particles, false);
+    return Vector3D(d.l[0], d.l[1], d.l[2]);
+}
+inline Vector3D centerOfMass(const std::vector<Particle>& particles) {
+    const nb_diag d = nbody::diagnostics(particles, false);
+    return Vector3D(d.com[0], d.com[1], d.com[2]);
+}
+inline double positionError(const std::vector<Particle>& a, const std::vector<Particle>& b) {
+    double err = 0.0;
+    for (size_t i = 0; i < a.size(); i++) err += (a[i].position - b[i].position).norm();
+    return err / a.size();
+}
+inline double velocityError(const std::vector<Particle>& a, const std::vector<Particle>& b) {
+    double err = 0.0;
+    for (size_t i = 0; i < a.size(); i++) err += (a[i].velocity - b[i].velocity).norm();
+    return err / a.size();
+}
+
+namespace nbody {
+// Trajectory rows in the column layout of the reference's orbit.csv
+// (CPU/src/n-body.cpp:6,19-58): the positions of two bodies (`body`, then
+// `center`), kinetic, potential and total energy, angular momentum and centre
+// of mass, printed fixed with 15 decimals.  The reference prints its first row
+// with the stream's default 6 significant digits; every row here uses 15
+// decimals.  Diagnostics are reduced on the GPU from device-resident state.
+class OrbitCsvWriter {
+   public:
+    OrbitCsvWriter(std::ostream& out, int64_t body, int64_t center, double G = constant::G)
+        : out_(out), body_(body), center_(center), G_(G) {
+        out_ << "earth_x,earth_y,earth_z,sun_x,sun_y,sun_z,kinetic_energy,potential_energy,"
+                "total_energy,angular_momentum_x,angular_momentum_y,angular_momentum_z,"
+                "center_of_mass_x,center_of_mass_y,center_of_mass_z\n";
+    }
+    void row(const ParticleArrays& p) {
+        std::vector<double> x(p.n), y(p.n), z(p.n);
+        check(nb_particles_download(&p, x.data(), y.data(), z.data(), nullptr, nullptr, nullptr,
+                                    nullptr, nullptr, nullptr, nullptr),
+              "OrbitCsvWriter");
+        nb_diag d{};
+        check(nb_diagnostics(&p, G_, 0.0, 1, &d), "OrbitCsvWriter");
+        const double v[15] = {x[body_], y[body_], z[body_], x[center_], y[center_], z[center_],
+                              d.ke, d.pe, d.ke + d.pe, d.l[0], d.l[1], d.l[2],
+                              d.com[0], d.com[1], d.com[2]};
+        out_ << std::fixed << std::setprecision(15);
+        for (int k = 0; k < 15; ++k) out_ << v[k] << (k < 14 ? ',' : '\n');
+    }
+
+   private:
+    std::ostream& out_;
+    int64_t body_, center_;
+    double G_;
+};
+}  // namespace nbody

@@ -1,7 +1,10 @@
-// nb_integrate.hip -- leapfrog (kick-drift-kick, cached acceleration) and RK4
-// on FP64 structure-of-arrays state.  The library is built with
-// -ffp-contract=off, so every update rounds exactly like the reference CPU
-// loops (CPU/src/integrator_leapfrog.cpp:8-21, CPU/src/integrator_rk4.cpp:18-42).
+// nb_integrate.hip -- leapfrog (kick-drift-kick, cached acceleration), RK4,
+// Euler, Euler-Cromer and velocity Verlet (cached acceleration) on FP64
+// structure-of-arrays state.  The library is built with -ffp-contract=off, so
+// every update rounds exactly like the reference CPU loops
+// (CPU/src/integrator_leapfrog.cpp:8-21, CPU/src/integrator_rk4.cpp:18-42,
+// CPU/src/integrator_euler.cpp:9-15, CPU/src/integrator_euler-cromer.cpp:9-15,
+// CPU/src/integrator_velocity_verlet.cpp:5-17).
 #include "nb_internal.hpp"
 
 struct nb_integrator {
@@ -9,7 +12,8 @@ struct nb_integrator {
     bool have_acc = false;           // leapfrog: acceleration of the current positions is cached
     const double* cached_x = nullptr; // state the cache belongs to
     int64_t cached_n = -1;
-    nb::DevBuf rk;                   // RK4 stage buffers (15 arrays of n doubles)
+    nb::DevBuf rk;                   // RK4 stage buffers (15 arrays of n doubles);
+                                     // velocity Verlet: the previous acceleration (3 arrays)
     std::string name;
 };
 
@@ -134,6 +138,59 @@ __global__ void k_rk4_fin(int n, Vec3Ptr x, Vec3Ptr v, CVec3Ptr a, double sixth,
 #undef NB_RK_F
 }
 
+// Euler: x = x + v*dt; v = v + a*dt  (CPU/src/integrator_euler.cpp:9-15,
+// CUDA/src/integrator_euler.cu:15-20).
+__global__ void k_euler(int n, Vec3Ptr x, Vec3Ptr v, CVec3Ptr a, double dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#define NB_EU(c)                          \
+    {                                     \
+        x.c[i] = x.c[i] + v.c[i] * dt;    \
+        v.c[i] = v.c[i] + a.c[i] * dt;    \
+    }
+    NB_EU(x) NB_EU(y) NB_EU(z)
+#undef NB_EU
+}
+
+// Euler-Cromer: v = v + a*dt; x = x + v*dt  (CPU/src/integrator_euler-cromer.cpp:9-15).
+__global__ void k_euler_cromer(int n, Vec3Ptr x, Vec3Ptr v, CVec3Ptr a, double dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#define NB_EC(c)                                  \
+    {                                             \
+        const double vn = v.c[i] + a.c[i] * dt;   \
+        v.c[i] = vn;                              \
+        x.c[i] = x.c[i] + vn * dt;                \
+    }
+    NB_EC(x) NB_EC(y) NB_EC(z)
+#undef NB_EC
+}
+
+// Velocity Verlet, first half (CPU/src/integrator_velocity_verlet.cpp:5-12):
+// x = x + ((v*dt) + (((a*0.5)*dt)*dt)); a_old = a.
+__global__ void k_vv_drift(int n, Vec3Ptr x, CVec3Ptr v, CVec3Ptr a, Vec3Ptr aold, double dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#define NB_VD(c)                                                        \
+    {                                                                   \
+        const double ac = a.c[i];                                       \
+        x.c[i] = x.c[i] + ((v.c[i] * dt) + (((ac * 0.5) * dt) * dt));   \
+        aold.c[i] = ac;                                                 \
+    }
+    NB_VD(x) NB_VD(y) NB_VD(z)
+#undef NB_VD
+}
+
+// Velocity Verlet, second half (after a = f(x)): v = v + ((a_old + a)*(0.5*dt))
+// (CPU/src/integrator_velocity_verlet.cpp:14-17).
+__global__ void k_vv_kick(int n, Vec3Ptr v, CVec3Ptr a, CVec3Ptr aold, double hdt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    v.x[i] = v.x[i] + ((aold.x[i] + a.x[i]) * hdt);
+    v.y[i] = v.y[i] + ((aold.y[i] + a.y[i]) * hdt);
+    v.z[i] = v.z[i] + ((aold.z[i] + a.z[i]) * hdt);
+}
+
 static Vec3Ptr pos(nb_particles* p) { return Vec3Ptr{p->x, p->y, p->z}; }
 static Vec3Ptr vel(nb_particles* p) { return Vec3Ptr{p->vx, p->vy, p->vz}; }
 static CVec3Ptr cacc(nb_particles* p) { return CVec3Ptr{p->ax, p->ay, p->az}; }
@@ -221,6 +278,49 @@ static int rk4(nb_integrator* it, nb_force* f, nb_particles* p, double dt, int s
     return NB_OK;
 }
 
+// Euler and Euler-Cromer evaluate the force at the start of every step, as
+// the reference does; velocity Verlet reuses the force of the previous step's
+// end (same positions, same bits as the reference's two evaluations per step).
+static int step1(nb_integrator* it, nb_force* f, nb_particles* p, double dt, int steps) {
+    const int64_t n = p->n;
+    if (n == 0 || steps <= 0) return NB_OK;
+    const int grid = grid_for(n, kIntBlock);
+    hipStream_t s = f->stream;
+    if (it->kind == NB_EULER || it->kind == NB_EULER_CROMER) {
+        for (int k = 0; k < steps; ++k) {
+            NB_TRY(nb_force_compute(f, p));
+            if (it->kind == NB_EULER)
+                hipLaunchKernelGGL(k_euler, dim3(grid), dim3(kIntBlock), 0, s, (int)n, pos(p),
+                                   vel(p), cacc(p), dt);
+            else
+                hipLaunchKernelGGL(k_euler_cromer, dim3(grid), dim3(kIntBlock), 0, s, (int)n,
+                                   pos(p), vel(p), cacc(p), dt);
+            NB_LAUNCH_CHECK();
+        }
+        it->have_acc = false;
+        return NB_OK;
+    }
+    const size_t stride = ((size_t)n * sizeof(double) + 255) & ~(size_t)255;
+    NB_TRY(it->rk.reserve(stride * 3));
+    char* b = it->rk.as<char>();
+    Vec3Ptr old{reinterpret_cast<double*>(b), reinterpret_cast<double*>(b + stride),
+                reinterpret_cast<double*>(b + 2 * stride)};
+    CVec3Ptr cold{old.x, old.y, old.z};
+    CVec3Ptr cv{p->vx, p->vy, p->vz};
+    const double hdt = 0.5 * dt;
+    NB_TRY(ensure_acc(it, f, p));
+    for (int k = 0; k < steps; ++k) {
+        hipLaunchKernelGGL(k_vv_drift, dim3(grid), dim3(kIntBlock), 0, s, (int)n, pos(p), cv,
+                           cacc(p), old, dt);
+        NB_LAUNCH_CHECK();
+        NB_TRY(nb_force_compute(f, p));
+        hipLaunchKernelGGL(k_vv_kick, dim3(grid), dim3(kIntBlock), 0, s, (int)n, vel(p), cacc(p),
+                           cold, hdt);
+        NB_LAUNCH_CHECK();
+    }
+    return NB_OK;
+}
+
 }  // namespace nb
 
 using namespace nb;
@@ -228,13 +328,15 @@ using namespace nb;
 extern "C" {
 
 int nb_integrator_create(nb_integrator** out, int kind) {
-    if (!out || (kind != NB_LEAPFROG && kind != NB_RK4)) {
+    static const char* const names[] = {"Leapfrog", "RK4", "Euler", "Euler-Cromer",
+                                        "Velocity-Verlet"};
+    if (!out || kind < NB_LEAPFROG || kind > NB_VELOCITY_VERLET) {
         set_error("nb_integrator_create: unknown kind %d", kind);
         return NB_ERR_INVALID;
     }
     nb_integrator* it = new nb_integrator();
     it->kind = kind;
-    it->name = kind == NB_LEAPFROG ? "Leapfrog" : "RK4";
+    it->name = names[kind];
     *out = it;
     return NB_OK;
 }
@@ -261,7 +363,11 @@ int nb_integrator_run(nb_integrator* it, nb_force* f, nb_particles* p, double dt
         set_error("nb_integrator_run: negative step count");
         return NB_ERR_INVALID;
     }
-    return it->kind == NB_LEAPFROG ? leapfrog(it, f, p, dt, steps) : rk4(it, f, p, dt, steps);
+    switch (it->kind) {
+        case NB_LEAPFROG: return leapfrog(it, f, p, dt, steps);
+        case NB_RK4: return rk4(it, f, p, dt, steps);
+        default: return step1(it, f, p, dt, steps);
+    }
 }
 
 const char* nb_integrator_name(const nb_integrator* it) { return it ? it->name.c_str() : ""; }

@@ -31,7 +31,7 @@ G_REF = 4.0 * math.pi * math.pi
 
 NB_DIRECT_F64_EXACT, NB_DIRECT_F64, NB_DIRECT_F32 = 0, 1, 2
 NB_MONOPOLE, NB_QUADRUPOLE = 0, 1
-NB_LEAPFROG, NB_RK4 = 0, 1
+NB_LEAPFROG, NB_RK4, NB_EULER, NB_EULER_CROMER, NB_VELOCITY_VERLET = 0, 1, 2, 3, 4
 STATUS = {0: "NB_OK", 1: "NB_ERR_INVALID", 2: "NB_ERR_HIP", 3: "NB_ERR_NOMEM",
           4: "NB_ERR_CAPACITY", 5: "NB_ERR_BUDGET", 6: "NB_ERR_STATE"}
 PRECISIONS = {"f64_exact": NB_DIRECT_F64_EXACT, "f64": NB_DIRECT_F64, "f32": NB_DIRECT_F32}
@@ -367,6 +367,18 @@ class RK4Integrator(TimeIntegrator):
     _kind = NB_RK4
 
 
+class EulerIntegrator(TimeIntegrator):
+    _kind = NB_EULER
+
+
+class EulerCromerIntegrator(TimeIntegrator):
+    _kind = NB_EULER_CROMER
+
+
+class VelocityVerletIntegrator(TimeIntegrator):
+    _kind = NB_VELOCITY_VERLET
+
+
 # --------------------------------------------------------------------------- diagnostics
 def diagnostics(particles: Particles, G: float = G_REF, softening: float = 0.0,
                 with_potential: bool = True) -> dict:
@@ -375,6 +387,34 @@ def diagnostics(particles: Particles, G: float = G_REF, softening: float = 0.0,
                               C.byref(d)))
     return dict(ke=d.ke, pe=d.pe, pe_soft=d.pe_soft, p=np.array(d.p[:]), l=np.array(d.l[:]),
                 com=np.array(d.com[:]), mass=d.mass)
+
+
+ORBIT_CSV_HEADER = ("earth_x,earth_y,earth_z,sun_x,sun_y,sun_z,kinetic_energy,potential_energy,"
+                    "total_energy,angular_momentum_x,angular_momentum_y,angular_momentum_z,"
+                    "center_of_mass_x,center_of_mass_y,center_of_mass_z")
+
+
+class OrbitCsvWriter:
+    """Trajectory rows in the column layout of the reference's orbit.csv
+    (CPU/src/n-body.cpp:6,19-58): positions of `body` then `center`, kinetic,
+    potential (unsoftened) and total energy, angular momentum and centre of
+    mass, fixed with 15 decimals.  Diagnostics are reduced on the GPU."""
+
+    def __init__(self, path, body=1, center=0, G=G_REF):
+        self._f = open(path, "w")
+        self._f.write(ORBIT_CSV_HEADER + "\n")
+        self.body, self.center, self.G = body, center, G
+
+    def row(self, particles: Particles):
+        g = particles.download(("x", "y", "z"))
+        d = diagnostics(particles, self.G, 0.0, True)
+        b, c = self.body, self.center
+        vals = [g["x"][b], g["y"][b], g["z"][b], g["x"][c], g["y"][c], g["z"][c],
+                d["ke"], d["pe"], d["ke"] + d["pe"], *d["l"], *d["com"]]
+        self._f.write(",".join("%.15f" % v for v in vals) + "\n")
+
+    def close(self):
+        self._f.close()
 
 
 # --------------------------------------------------------------------------- initial conditions

@@ -8,7 +8,8 @@
 // this file is therefore compiled with -ffp-contract=off.
 //
 // What pins each part (checked in tests/test_oracle_*.py):
-//   direct / leapfrog / rk4       bit-exact hashes from the compiled reference
+//   direct / leapfrog / rk4 /    bit-exact hashes from the compiled reference
+//   euler / euler-cromer / vv
 //                                 (tests/golden/ref_golden.json) and the 200
 //                                 error lines of CPU/outputs/output2_4.txt
 //   bhopt (topology + monopole)   compiled-reference hashes + output2_4.txt
@@ -490,6 +491,49 @@ void orc_rk4_direct(int64_t n, double G, double soft2, double dt, int steps, con
             vx[i] = vx[i] + (((ax[i] + 2.0 * ax2[i]) + 2.0 * ax3[i]) + ax4[i]) * sixth;
             vy[i] = vy[i] + (((ay[i] + 2.0 * ay2[i]) + 2.0 * ay3[i]) + ay4[i]) * sixth;
             vz[i] = vz[i] + (((az[i] + 2.0 * az2[i]) + 2.0 * az3[i]) + az4[i]) * sixth;
+        }
+    }
+}
+
+// First-order and velocity-Verlet integrators with the direct force, in the
+// CPU form: the force is evaluated at the start of every step.
+// kind 0 = Euler (CPU/src/integrator_euler.cpp:3-15):
+//   a = f(x); x = x + v*dt; v = v + a*dt.
+// kind 1 = Euler-Cromer (CPU/src/integrator_euler-cromer.cpp:3-15):
+//   a = f(x); v = v + a*dt; x = x + v*dt.
+// kind 2 = velocity Verlet (CPU/src/integrator_velocity_verlet.cpp:3-23):
+//   a = f(x); x = x + ((v*dt) + (((a*0.5)*dt)*dt)); a_old = a; a = f(x);
+//   v = v + ((a_old + a)*(0.5*dt)).
+// On return a* holds the last force evaluated (the reference leaves it there).
+void orc_step1_direct(int64_t n, int kind, double G, double soft2, double dt, int steps,
+                      const double* m, double* x, double* y, double* z, double* vx, double* vy,
+                      double* vz, double* ax, double* ay, double* az) {
+    std::vector<double> ox(n), oy(n), oz(n);
+    const double hdt = 0.5 * dt;
+    for (int k = 0; k < steps; ++k) {
+        SoA s{n, m, x, y, z};
+        direct_into(s, G, soft2, ax, ay, az);
+        if (kind == 0) {
+            drift(n, x, y, z, vx, vy, vz, dt);
+            kick(n, vx, vy, vz, ax, ay, az, dt);
+        } else if (kind == 1) {
+            kick(n, vx, vy, vz, ax, ay, az, dt);
+            drift(n, x, y, z, vx, vy, vz, dt);
+        } else {
+            for (int64_t i = 0; i < n; ++i) {
+                x[i] = x[i] + ((vx[i] * dt) + (((ax[i] * 0.5) * dt) * dt));
+                y[i] = y[i] + ((vy[i] * dt) + (((ay[i] * 0.5) * dt) * dt));
+                z[i] = z[i] + ((vz[i] * dt) + (((az[i] * 0.5) * dt) * dt));
+                ox[i] = ax[i];
+                oy[i] = ay[i];
+                oz[i] = az[i];
+            }
+            direct_into(s, G, soft2, ax, ay, az);
+            for (int64_t i = 0; i < n; ++i) {
+                vx[i] = vx[i] + ((ox[i] + ax[i]) * hdt);
+                vy[i] = vy[i] + ((oy[i] + ay[i]) * hdt);
+                vz[i] = vz[i] + ((oz[i] + az[i]) * hdt);
+            }
         }
     }
 }

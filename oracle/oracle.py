@@ -42,6 +42,7 @@ def lib():
         L.orc_direct.argtypes = [_i64, _d, _d, _D, _D, _D, _D, _D, _D, _D]
         L.orc_leapfrog_direct.argtypes = [_i64, _d, _d, _d, _i, _i, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
         L.orc_rk4_direct.argtypes = [_i64, _d, _d, _d, _i, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
+        L.orc_step1_direct.argtypes = [_i64, _i, _d, _d, _d, _i] + [_D] * 10
         L.orc_bhopt.argtypes = [_i64, _d, _d, _d, _i, _D, _D, _D, _D, _D, _D, _D, C.c_void_p]
         L.orc_bhopt_leaves.argtypes = [_i64, _i, _D, _D, _D, _D, _I32, _U64, C.POINTER(C.c_int64)]
         L.orc_bhopt_leaves.restype = C.c_int64
@@ -104,6 +105,19 @@ def rk4_direct(m, x, y, z, vx, vy, vz, dt, steps, G=G_REF, soft2=0.0):
     n = len(m)
     acc = [np.zeros(n) for _ in range(3)]
     lib().orc_rk4_direct(n, G, soft2, dt, steps, m, *st, *acc)
+    return st + acc
+
+
+STEP1_KINDS = {"euler": 0, "euler_cromer": 1, "velocity_verlet": 2}
+
+
+def step1_direct(kind, m, x, y, z, vx, vy, vz, dt, steps, G=G_REF, soft2=0.0):
+    """Euler / Euler-Cromer / velocity Verlet with the direct force (CPU form)."""
+    m = _c(m)
+    st = [_c(a).copy() for a in (x, y, z, vx, vy, vz)]
+    n = len(m)
+    acc = [np.zeros(n) for _ in range(3)]
+    lib().orc_step1_direct(n, STEP1_KINDS[kind], G, soft2, dt, steps, m, *st, *acc)
     return st + acc
 
 
@@ -189,7 +203,7 @@ def ref_integrate_direct(integrator, m, x, y, z, vx, vy, vz, dt, steps, softenin
     st = [_c(a).copy() for a in (x, y, z, vx, vy, vz)]
     n = len(m)
     acc = [np.zeros(n) for _ in range(3)]
-    code = {"leapfrog": 0, "rk4": 1}[integrator]
+    code = {"leapfrog": 0, "rk4": 1, "euler": 2, "euler_cromer": 3, "velocity_verlet": 4}[integrator]
     ref().ref_integrate_direct(n, code, softening, dt, steps, m, *st, *acc)
     return st + acc
 

@@ -17,6 +17,9 @@
 #include "force_calculator_naive.hpp"
 #include "integrator_leapfrog.hpp"
 #include "integrator_rk4.hpp"
+#include "integrator_euler.hpp"
+#include "integrator_euler-cromer.hpp"
+#include "integrator_velocity_verlet.hpp"
 
 namespace {
 
@@ -114,18 +117,31 @@ void ref_direct(int64_t n, double softening, const double* m, const double* x, c
     store(p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ax, ay, az);
 }
 
-// integrator: 0 = LeapfrogIntegrator, 1 = RK4Integrator; force: ForceCalculatorNaive.
+// integrator: 0 = LeapfrogIntegrator, 1 = RK4Integrator, 2 = EulerIntegrator,
+// 3 = EulerCromerIntegrator, 4 = VelocityVerletIntegrator; force: ForceCalculatorNaive.
 void ref_integrate_direct(int64_t n, int integrator, double softening, double dt, int steps,
                           const double* m, double* x, double* y, double* z, double* vx, double* vy,
                           double* vz, double* ax, double* ay, double* az) {
     auto p = make(n, m, x, y, z, vx, vy, vz);
     ForceCalculatorNaive f(softening);
+    auto run = [&](TimeIntegrator& it) {
+        for (int k = 0; k < steps; ++k) it.step(p, f, dt);
+    };
     if (integrator == 0) {
         LeapfrogIntegrator it;
-        for (int k = 0; k < steps; ++k) it.step(p, f, dt);
-    } else {
+        run(it);
+    } else if (integrator == 1) {
         RK4Integrator it;
-        for (int k = 0; k < steps; ++k) it.step(p, f, dt);
+        run(it);
+    } else if (integrator == 2) {
+        EulerIntegrator it;
+        run(it);
+    } else if (integrator == 3) {
+        EulerCromerIntegrator it;
+        run(it);
+    } else {
+        VelocityVerletIntegrator it;
+        run(it);
     }
     store(p, x, y, z, vx, vy, vz, ax, ay, az);
 }

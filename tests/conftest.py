@@ -35,6 +35,13 @@ def golden():
         return json.load(f)
 
 
+@pytest.fixture(scope="session")
+def orbit_rows():
+    """Header and sampled rows of the reference's CPU/orbit.csv."""
+    with open(os.path.join(GOLDEN, "orbit_csv_rows.json")) as f:
+        return json.load(f)
+
+
 def load_fixture(name):
     """Particle file from tests/golden (reference layout: N, then m x y z vx vy vz)."""
     with open(os.path.join(GOLDEN, name)) as f:

@@ -9,6 +9,8 @@ Outputs (data only -- inputs, hashes and expected values):
                          a few %.17g sample values per case
   output2_4_errors.json  the 200 error statistics printed by the reference's
                          CPU/tests/test2_ForceBenchmark.cpp in CPU/outputs/output2_4.txt
+  orbit_csv_rows.json    header and a sample of rows of the reference's CPU/orbit.csv
+                         (Earth-Sun, velocity Verlet, dt = 1/365, printed by CPU/src/n-body.cpp)
   config1_n1000.txt      the config-1-shaped input (N=1000, m=1/N, x~U[-1,1]^3,
                          v~N(0,0.1^2) per axis) in the particle-file layout
 """
@@ -75,8 +77,20 @@ def parse_output2_4():
     return res
 
 
+ORBIT_ROWS = (0, 1, 2, 3, 10, 50, 91, 100, 182, 183, 250, 300, 364, 365, 366)
+
+
+def orbit_rows():
+    with open(os.path.join(REF, "CPU/orbit.csv")) as f:
+        lines = f.read().splitlines()
+    return {"header": lines[0], "n_rows": len(lines) - 1,
+            "rows": {str(i): lines[1 + i] for i in ORBIT_ROWS}}
+
+
 def main():
     assert O.have_ref(), "build the compiled reference first: make -C oracle ref"
+    with open(os.path.join(HERE, "orbit_csv_rows.json"), "w") as f:
+        json.dump(orbit_rows(), f, indent=1)
     gold = {}
     fixtures = {"p100": load(os.path.join(HERE, "particles_100.txt")),
                 "p1000": load(os.path.join(HERE, "particles_1000.txt"))}
@@ -106,6 +120,13 @@ def main():
                                     d["vz"], 1e-3, steps, 0.01)
         gold[f"rk4/{fx}/eps0.01/dt1e-3/steps{steps}"] = {"sha256": digest(*st),
                                                           "sample": sample(st, st_names)}
+    for integ in ("euler", "euler_cromer", "velocity_verlet"):
+        for fx, steps in (("p1000", 10), ("c1", 20)):
+            d = fixtures[fx]
+            st = O.ref_integrate_direct(integ, d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"],
+                                        d["vz"], 1e-3, steps, 0.01)
+            gold[f"{integ}/{fx}/eps0.01/dt1e-3/steps{steps}"] = {"sha256": digest(*st),
+                                                                  "sample": sample(st, st_names)}
     for fx in ("p100", "p1000", "c1"):
         d = fixtures[fx]
         for soft in (0.0, 0.005):

@@ -231,3 +231,42 @@ def test_plummer_1m_full_size(nb, O):
     assert np.array_equal(top["depth"], dep) and np.array_equal(top["path"], path)
     e = rel_err(p.acc(), f64_direct(nb, pl, 0.005, 1.0))
     assert np.percentile(e, 99) <= 1e-3 and np.median(e) <= 3e-4
+
+
+def test_leaf_capacity_sweep_harness(O):
+    """The leaf-capacity / theta sweep (counterpart of
+    CPU/tests/test3_computeLeafCapacity.cpp) on the reference's own inputs:
+    it prints both tables and one JSON summary per particle count, every
+    theta = 0.5 configuration is more accurate than the reference's monopole
+    BH at theta = 0.5 on the same input, and errors grow with theta."""
+    import json
+    import os
+    import subprocess
+
+    from conftest import ROOT
+
+    exe = os.path.join(ROOT, "n-body-gravity-simulation_amd", "bin", "leaf_capacity_sweep")
+    assert os.path.exists(exe), "build the adapter examples first"
+    r = subprocess.run([exe, "--n", "1000,5000", "--caps", "1,8,64", "--theta", "0.3,0.5,0.8",
+                        "--experiments", "2", "--iterations", "3"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("Fastest") == 2 and r.stdout.count("Most Accurate") == 2
+    summaries = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+    assert [s["n"] for s in summaries] == [1000, 5000]
+    for s in summaries:
+        res = {(t, c): (ms, avg, mx) for t, c, ms, avg, mx in s["results"]}
+        assert len(res) == 9
+        # The reference's monopole BH (cap 8, theta 0.5) on experiment 0's input.
+        m, x, y, z = O.mt19937_particles(s["n"], order=1, seed=42)
+        ref_bh = O.bhopt(m, x, y, z, 0.5, 8, O.G_REF, 0.0)
+        direct = O.direct(m, x, y, z, O.G_REF, 0.0)
+        a_ref = np.stack(direct, 1)
+        rel = np.linalg.norm(np.stack(ref_bh, 1) - a_ref, axis=1) / np.linalg.norm(a_ref, axis=1)
+        for c in (1, 8, 64):
+            assert res[(0.5, c)][1] < rel.mean(), (s["n"], c)
+            assert res[(0.3, c)][1] <= res[(0.5, c)][1] <= res[(0.8, c)][1], (s["n"], c)
+            assert res[(0.3, c)][1] < res[(0.8, c)][1], (s["n"], c)
+        if s["leaf_capacity"] is not None:
+            assert s["avg_rel_error"] < s["max_error"]
+

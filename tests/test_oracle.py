@@ -165,6 +165,25 @@ def test_direct_momentum_conservation(O, fixtures):
     assert np.abs(f.sum(0)).max() < 1e-12 * np.abs(f).sum()
 
 
+@pytest.mark.parametrize("integ", ["euler", "euler_cromer", "velocity_verlet"])
+@pytest.mark.parametrize("fx,steps", [("p1000", 10), ("c1", 20)])
+def test_step1_integrators_match_compiled_reference(O, golden, fixtures, integ, fx, steps):
+    d = fixtures[fx]
+    st = O.step1_direct(integ, d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], 1e-3,
+                        steps, G_REF, 1e-4)
+    assert sha256(*st) == golden[f"{integ}/{fx}/eps0.01/dt1e-3/steps{steps}"]["sha256"]
+
+
+def test_velocity_verlet_time_reversible(O, fixtures):
+    d = fixtures["c1"]
+    st = O.step1_direct("velocity_verlet", d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"],
+                        d["vz"], 1e-3, 20, 1.0, 1e-4)
+    back = O.step1_direct("velocity_verlet", d["m"], st[0], st[1], st[2], -st[3], -st[4], -st[5],
+                          1e-3, 20, 1.0, 1e-4)
+    for a, b in zip(back[:3], (d["x"], d["y"], d["z"])):
+        assert np.abs(a - b).max() < 1e-12
+
+
 def test_leapfrog_time_reversible(O, fixtures):
     d = fixtures["c1"]
     st = O.leapfrog_direct(d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], 1e-3, 20,
@@ -175,11 +194,13 @@ def test_leapfrog_time_reversible(O, fixtures):
         assert np.abs(a - b).max() < 1e-12
 
 
-@pytest.mark.parametrize("integ,order", [("leapfrog", 2), ("rk4", 4)])
+@pytest.mark.parametrize("integ,order", [("leapfrog", 2), ("rk4", 4), ("euler", 1),
+                                         ("euler_cromer", 1), ("velocity_verlet", 2)])
 def test_kepler_convergence_order(O, integ, order):
     """Earth on a circular orbit around the Sun (G=4pi^2, 1 yr period,
-    CPU/tests/test1_EarthSun.cpp): the position error after one period falls as
-    dt^order."""
+    CPU/tests/test1_EarthSun.cpp): the position error after a quarter period
+    falls as dt^order.  (A quarter, not a whole period: over a whole circular
+    period the first-order error of Euler-Cromer cancels.)"""
     me = 3.003e-6
     m = np.array([1.0, me])
     v = 2 * math.pi * math.sqrt(1 + me)
@@ -189,12 +210,40 @@ def test_kepler_convergence_order(O, integ, order):
     errs = []
     for steps in (200, 400):
         dt = period / steps
-        fn = O.leapfrog_direct if integ == "leapfrog" else O.rk4_direct
-        st = fn(m, x0, y0, z0, vx0, vy0, vz0, dt, steps, G_REF, 0.0)
+        n = steps // 4
+        if integ in O.STEP1_KINDS:
+            st = O.step1_direct(integ, m, x0, y0, z0, vx0, vy0, vz0, dt, n, G_REF, 0.0)
+        else:
+            fn = O.leapfrog_direct if integ == "leapfrog" else O.rk4_direct
+            st = fn(m, x0, y0, z0, vx0, vy0, vz0, dt, n, G_REF, 0.0)
         rx = st[0][1] - st[0][0]; ry = st[1][1] - st[1][0]
-        errs.append(math.hypot(rx - 1.0, ry))
+        errs.append(math.hypot(rx, ry - 1.0))  # a quarter turn from (1, 0) is (0, 1)
     ratio = errs[0] / errs[1]
     assert 2 ** order * 0.7 < ratio < 2 ** order * 1.3, (errs, ratio)
+
+
+def test_velocity_verlet_reproduces_reference_orbit_csv(O, orbit_rows):
+    """The Earth-Sun run of CPU/src/n-body.cpp (velocity Verlet, dt = 1/365,
+    G = 4 pi^2): positions and diagnostics equal the rows of the reference's
+    committed CPU/orbit.csv to its printed precision (15 decimals; the first
+    row is printed with 6 significant digits)."""
+    m = np.array([1.0, 3.003e-6])
+    st = [np.array([0.0, 1.0]), np.zeros(2), np.zeros(2), np.zeros(2),
+          np.array([0.0, 2 * math.pi]), np.zeros(2)]
+    dt = 1.0 / 365.0
+    done = 0
+    for r in sorted(int(k) for k in orbit_rows["rows"]):
+        if r > done:
+            st = O.step1_direct("velocity_verlet", m, *st, dt, r - done, G_REF, 0.0)[:6]
+            done = r
+        ref = np.array([float(v) for v in orbit_rows["rows"][str(r)].split(",")])
+        d = O.diag(m, *st, G=G_REF, soft2=0.0)
+        mine = np.array([st[0][1], st[1][1], st[2][1], st[0][0], st[1][0], st[2][0], d["ke"],
+                         d["pe"], d["ke"] + d["pe"], *d["l"], *d["com"]])
+        if r == 0:
+            assert np.allclose(mine, ref, rtol=5e-6, atol=0), r
+        else:
+            assert np.abs(mine - ref).max() <= 6e-16, r
 
 
 def test_diag_two_body(O):
