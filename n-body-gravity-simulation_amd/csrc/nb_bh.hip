@@ -50,7 +50,7 @@ constexpr int kSortBins = 256;
 constexpr int kSortPasses = 8;
 constexpr int kWalkFcap = 2048;  // walk frontier capacity per block (opened cells per level)
 constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
-constexpr int kUpsweepGrid = 512;
+constexpr int kUpsweepGrid = 128;  // blocks per internal-level up-sweep launch
 constexpr int kGroup = 64;      // walk group: cells of at most 64 particles
 // Per-block count columns: 22 levels, group starts, leaf starts, oversize leaves.
 constexpr int kColGroup = kLevels, kColLeaf = kLevels + 1, kColOver = kLevels + 2;
@@ -88,7 +88,7 @@ struct nb::BHWorkspace {
     int64_t last_n = 0;
     int last_cap = 8;
     int last_valid = 0;
-    DevBuf root, bbox_part, keys0, keys1, vals0, vals1, hist, spos;
+    DevBuf root, bbox_part, keys0, keys1, vals0, vals1, hist, spos, upos;
     DevBuf lvl_leaf, lvl_sh, blkcnt, blkoff, levels, leaf_of, counters;
     DevBuf n_pstart, n_pend, n_child, n_cend, n_level, n_mom, w_mac, w_link, w_q0, w_q1;
     DevBuf walk_scratch, gflag, gstart, glev, totals;
@@ -123,7 +123,7 @@ __device__ __forceinline__ int shared_levels(unsigned long long a, unsigned long
 }
 
 // ------------------------------------------------------------------ counters
-__global__ void k_reset_counters(Counters* c) {
+__device__ __forceinline__ void reset_counters(Counters* c) {
     c->pp = 0;
     c->pc = 0;
     c->max_frontier = 0;
@@ -170,9 +170,11 @@ __global__ __launch_bounds__(kBlk) void k_bbox_partial(int n, const double* __re
 }
 
 __global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* __restrict__ part,
-                                                     RootInfo* __restrict__ root) {
+                                                     RootInfo* __restrict__ root,
+                                                     Counters* __restrict__ ctr) {
     __shared__ double sh[6][kBlk];
     const int t = threadIdx.x;
+    if (t == 0) reset_counters(ctr);
     double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (int k = t; k < nparts; k += kBlk) {
 #pragma unroll
@@ -219,17 +221,22 @@ __global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* _
 }
 
 // ------------------------------------------------------------------ keys
+// Also writes the FP32 (x, y, z, m) of every particle in original order, so
+// the gather after the sort moves one 16-byte record per particle.
 __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__ x,
                                                const double* __restrict__ y,
                                                const double* __restrict__ z,
+                                               const double* __restrict__ m,
                                                const RootInfo* __restrict__ root,
                                                unsigned long long* __restrict__ keys,
-                                               unsigned* __restrict__ vals) {
+                                               unsigned* __restrict__ vals,
+                                               float4* __restrict__ upos) {
     const int i = blockIdx.x * kBlk + threadIdx.x;
     if (i >= n) return;
     double cx = root->center[0], cy = root->center[1], cz = root->center[2];
     double s = root->size;
     const double px = x[i], py = y[i], pz = z[i];
+    upos[i] = make_float4((float)px, (float)py, (float)pz, (float)m[i]);
     unsigned long long k = 0;
 #pragma unroll
     for (int l = 0; l < kMaxDigits; ++l) {
@@ -363,15 +370,11 @@ __global__ __launch_bounds__(kBlk) void k_radix_scatter(
 }
 
 __global__ __launch_bounds__(kBlk) void k_gather(int n, const unsigned* __restrict__ perm,
-                                                 const double* __restrict__ x,
-                                                 const double* __restrict__ y,
-                                                 const double* __restrict__ z,
-                                                 const double* __restrict__ m,
+                                                 const float4* __restrict__ upos,
                                                  float4* __restrict__ spos) {
     const int k = blockIdx.x * kBlk + threadIdx.x;
     if (k >= n) return;
-    const unsigned j = perm[k];
-    spos[k] = make_float4((float)x[j], (float)y[j], (float)z[j], (float)m[j]);
+    spos[k] = upos[perm[k]];
 }
 
 // ------------------------------------------------------------------ build
@@ -673,11 +676,26 @@ __device__ void node_moments(int id, int n, int total, const RootInfo* root,
     q1[id] = make_float4((float)Q[3], (float)Q[4], (float)Q[5], 0.f);
 }
 
-// Level-synchronous up-sweep: one launch per level, deepest level first, so
-// every child is finished (and visible at the kernel boundary) before its
-// parent is computed.  (A single-launch version with per-node arrival
-// counters needs a device-scope release/acquire per node, which on this GPU
-// writes back / invalidates the XCD-local L2 and was 5x slower.)
+// Leaves depend on their particles only: all of them in one launch.
+__global__ __launch_bounds__(kBlk) void k_leaf_moments(
+    const LevelInfo* __restrict__ li, const RootInfo* __restrict__ root,
+    const int* __restrict__ pstart, const int* __restrict__ pend, const int* child,
+    const int* cend, const signed char* __restrict__ level, const float4* __restrict__ spos,
+    double* mom, float4* mac, int2* link, float4* q0, float4* q1, double inv_theta2, int quad,
+    int n, Counters* ctr) {
+    if (ctr->overflow & 6) return;
+    const int total = li->total;
+    for (int id = blockIdx.x * kBlk + threadIdx.x; id < total; id += gridDim.x * kBlk)
+        if (child[id] < 0)
+            node_moments(id, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
+                         link, q0, q1, inv_theta2, quad, ctr);
+}
+
+// Internal nodes, level-synchronous: one launch per level, deepest level
+// first, so every child is finished (and visible at the kernel boundary)
+// before its parent is computed.  (A single-launch version with per-node
+// arrival counters needs a device-scope release/acquire per node, which on
+// this GPU writes back / invalidates the XCD-local L2 and was 5x slower.)
 __global__ __launch_bounds__(kBlk) void k_upsweep(
     int l, const LevelInfo* __restrict__ li, const RootInfo* __restrict__ root,
     const int* __restrict__ pstart, const int* __restrict__ pend, const int* child,
@@ -689,8 +707,9 @@ __global__ __launch_bounds__(kBlk) void k_upsweep(
     const int st = li->start[l];
     const int total = li->total;
     for (int k = blockIdx.x * kBlk + threadIdx.x; k < cnt; k += gridDim.x * kBlk)
-        node_moments(st + k, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
-                     link, q0, q1, inv_theta2, quad, ctr);
+        if (child[st + k] >= 0)
+            node_moments(st + k, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
+                         link, q0, q1, inv_theta2, quad, ctr);
 }
 
 // ------------------------------------------------------------------ walk
@@ -1025,6 +1044,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_TRY(w->hist.reserve((size_t)nsort * kSortBins * 4));
     NB_TRY(w->totals.reserve((size_t)std::max(kSortBins, kCols) * 4));
     NB_TRY(w->spos.reserve((size_t)n * 16));
+    NB_TRY(w->upos.reserve((size_t)n * 16));
     NB_TRY(w->lvl_leaf.reserve((size_t)n));
     NB_TRY(w->lvl_sh.reserve((size_t)n));
     NB_TRY(w->blkcnt.reserve((size_t)nblk * kCols * 4));
@@ -1051,13 +1071,13 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     auto* v0 = w->vals0.as<unsigned>();
     auto* v1 = w->vals1.as<unsigned>();
 
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, s, ctr);
     hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(kBlk), 0, s, n, x, y, z,
                        w->bbox_part.as<double>());
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlk), 0, s, nbb, w->bbox_part.as<double>(),
-                       root);
+                       root, ctr);
     if (ev && ev[1]) NB_HIP(hipEventRecord(ev[1], s));
-    hipLaunchKernelGGL(k_keys, dim3(nblk), dim3(kBlk), 0, s, n, x, y, z, root, k0, v0);
+    hipLaunchKernelGGL(k_keys, dim3(nblk), dim3(kBlk), 0, s, n, x, y, z, m, root, k0, v0,
+                       w->upos.as<float4>());
     NB_LAUNCH_CHECK();
     if (ev && ev[2]) NB_HIP(hipEventRecord(ev[2], s));
     for (int p = 0; p < kSortPasses; ++p) {
@@ -1074,7 +1094,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_LAUNCH_CHECK();
     // kSortPasses is even: the sorted keys are back in keys0/vals0.
     if (ev && ev[3]) NB_HIP(hipEventRecord(ev[3], s));
-    hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlk), 0, s, n, v0, x, y, z, m,
+    hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlk), 0, s, n, v0, w->upos.as<float4>(),
                        w->spos.as<float4>());
     if (ev && ev[4]) NB_HIP(hipEventRecord(ev[4], s));
     if (w->group_size == 0) {  // walk group size: NB_WALK_GROUP overrides the default
@@ -1098,6 +1118,12 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_LAUNCH_CHECK();
     if (ev && ev[5]) NB_HIP(hipEventRecord(ev[5], s));
     const double inv_theta2 = theta > 0.0 ? 1.0 / (theta * theta) : INFINITY;
+    hipLaunchKernelGGL(k_leaf_moments, dim3(grid_for(n, kBlk)), dim3(kBlk), 0, s, li, root,
+                       w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
+                       w->n_cend.as<int>(), w->n_level.as<signed char>(), w->spos.as<float4>(),
+                       w->n_mom.as<double>(), w->w_mac.as<float4>(), w->w_link.as<int2>(),
+                       w->w_q0.as<float4>(), w->w_q1.as<float4>(), inv_theta2,
+                       multipole == NB_QUADRUPOLE ? 1 : 0, n, ctr);
     for (int l = kLevels - 1; l >= 0; --l) {
         hipLaunchKernelGGL(k_upsweep, dim3(kUpsweepGrid), dim3(kBlk), 0, s, l, li, root,
                            w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
