@@ -821,20 +821,20 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
             const float inv = __builtin_amdgcn_rsqf(r2);
             const float inv2 = inv * inv;
-            const float inv3 = inv * inv2;
             if (kQuad) {
+                // a = d (M/r^3 + 5/2 dQd/r^7) - Qd/r^5
+                //   = (1/r^5) [d (M r^2 + 5/2 dQd/r^2) - Qd]
                 const float qdx = fmaf(qa.y, dx, fmaf(qb.x, dy, qb.y * dz));
                 const float qdy = fmaf(qb.x, dx, fmaf(qa.z, dy, qb.z * dz));
                 const float qdz = fmaf(qb.y, dx, fmaf(qb.z, dy, qa.w * dz));
                 const float dqd = fmaf(dx, qdx, fmaf(dy, qdy, dz * qdz));
-                const float inv5 = inv3 * inv2;
-                const float inv7 = inv5 * inv2;
-                const float cm = fmaf(2.5f * dqd, inv7, qa.x * inv3);
-                fx = fmaf(dx, cm, fmaf(-qdx, inv5, fx));
-                fy = fmaf(dy, cm, fmaf(-qdy, inv5, fy));
-                fz = fmaf(dz, cm, fmaf(-qdz, inv5, fz));
+                const float inv5 = (inv2 * inv2) * inv;
+                const float sc = fmaf(2.5f, dqd * inv2, qa.x * r2);
+                fx = fmaf(fmaf(dx, sc, -qdx), inv5, fx);
+                fy = fmaf(fmaf(dy, sc, -qdy), inv5, fy);
+                fz = fmaf(fmaf(dz, sc, -qdz), inv5, fz);
             } else {
-                const float cm = qa.x * inv3;
+                const float cm = qa.x * (inv * inv2);
                 fx = fmaf(dx, cm, fx);
                 fy = fmaf(dy, cm, fy);
                 fz = fmaf(dz, cm, fz);
