@@ -40,7 +40,8 @@ sys.path.insert(0, os.path.join(ROOT, "n-body-gravity-simulation_amd"))
 
 N_DEFAULT = 1 << 20
 THETA, EPS, DT, CAP, SEED = 0.5, 0.005, 1e-4, 8, 2
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector (packed) / dense FP32 MFMA
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # FP32 vector without packed instructions
 PEAK_HBM_GBS = 8000.0
 FLOP_PP, FLOP_PC_QUAD = 20, 60
 STAGE_EVALS = 3  # untimed evaluations for the per-stage breakdown
@@ -51,6 +52,34 @@ REF_PUBLISHED = 1_000_000 / 0.37334
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per
+    dispatch, each scaled by the factor a 256 MiB device copy calibrates), or
+    None when no measurement for this kernel is committed."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t["kernels"][kernel]
+        c = t["calibration"]
+        return {"bytes": int((k["fetch_kib"] * c["fetch_factor"] + k["write_kib"] * c["write_factor"])
+                             * 1024),
+                "source": "profiles/traffic.json (" + k["workload"] + ")"}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def traffic_bytes(kernel):
+    t = measured_traffic(kernel)
+    return None if t is None else t["bytes"]
+
+
+def traffic_source(kernel):
+    t = measured_traffic(kernel)
+    return None if t is None else t["source"]
 
 
 def stage_bytes(n, nodes):
@@ -139,7 +168,10 @@ def bench_direct(args):
                                          "N=100,000 (CUDA/outputs/output1_1.txt:130)"},
         "roofline": {"kernel": "k_direct_f32", "bound": "valu", "achieved": round(achieved, 2),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "frac_of_nonpacked_fp32": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 4),
+                     "traffic": traffic_bytes("k_direct_f32"),
+                     "traffic_source": traffic_source("k_direct_f32"),
                      "launch_ms": round(kern_ms, 4), "flops_per_launch": FLOP_PP * n * (n - 1)},
         "accuracy": {"median_rel_vs_f64": float(np.median(err)),
                      "p99_rel_vs_f64": float(np.percentile(err, 99)),
@@ -352,9 +384,12 @@ def main():
     flops = FLOP_PP * st["pp_interactions"] + FLOP_PC_QUAD * st["pc_interactions"]
     achieved = flops / (walk_ms * 1e-3) / 1e12
     roofline = {"kernel": "k_walk", "bound": "valu", "achieved": round(achieved, 2),
-                "peak_kind": "FP32 vector (157.3 TFLOP/s is also the dense FP32 MFMA figure)",
+                "peak_kind": "FP32 vector, packed (157.3 TFLOP/s is also the dense FP32 MFMA figure); "
+                             "this build runs non-packed FP32, whose peak is 78.6 TFLOP/s",
+                "frac_of_nonpacked_fp32": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 4),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": None, "launch_ms": round(walk_ms, 4),
+                "traffic": traffic_bytes("k_walk"), "traffic_source": traffic_source("k_walk"),
+                "launch_ms": round(walk_ms, 4),
                 "flops_per_launch": int(flops),
                 "interactions_per_particle": {"pp": round(st["pp_interactions"] / n, 1),
                                               "pc": round(st["pc_interactions"] / n, 1)}}
