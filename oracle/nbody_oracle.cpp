@@ -54,20 +54,24 @@ struct SoA {
 //   r = p_j - p_i; d2 = dot(r,r) + eps2; inv = 1/sqrt(d2);
 //   inv3 = (inv*inv)*inv; a_i += r * ((G*m_j)*inv3), j ascending, j != i.
 // ---------------------------------------------------------------------------
+V3 direct_row(const SoA& s, int64_t i, double G, double soft2) {
+    V3 pi = s.pos(i);
+    V3 acc{0.0, 0.0, 0.0};
+    for (int64_t j = 0; j < s.n; ++j) {
+        if (j == i) continue;
+        V3 r = sub(s.pos(j), pi);
+        double d2 = dot(r, r) + soft2;
+        double inv = 1.0 / std::sqrt(d2);
+        double inv3 = inv * inv * inv;
+        acc = add(acc, scale(r, G * s.m[j] * inv3));
+    }
+    return acc;
+}
+
 void direct_into(const SoA& s, double G, double soft2, double* ax, double* ay, double* az) {
-    const int64_t n = s.n;
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < n; ++i) {
-        V3 pi = s.pos(i);
-        V3 acc{0.0, 0.0, 0.0};
-        for (int64_t j = 0; j < n; ++j) {
-            if (j == i) continue;
-            V3 r = sub(s.pos(j), pi);
-            double d2 = dot(r, r) + soft2;
-            double inv = 1.0 / std::sqrt(d2);
-            double inv3 = inv * inv * inv;
-            acc = add(acc, scale(r, G * s.m[j] * inv3));
-        }
+    for (int64_t i = 0; i < s.n; ++i) {
+        const V3 acc = direct_row(s, i, G, soft2);
         ax[i] = acc.x;
         ay[i] = acc.y;
         az[i] = acc.z;
@@ -415,6 +419,20 @@ void orc_direct(int64_t n, double G, double soft2, const double* m, const double
                 const double* z, double* ax, double* ay, double* az) {
     SoA s{n, m, x, y, z};
     direct_into(s, G, soft2, ax, ay, az);
+}
+
+// The direct sum for the targets idx[0..k) only (same arithmetic per target).
+void orc_direct_subset(int64_t n, double G, double soft2, const double* m, const double* x,
+                       const double* y, const double* z, int64_t k, const int64_t* idx,
+                       double* ax, double* ay, double* az) {
+    SoA s{n, m, x, y, z};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t t = 0; t < k; ++t) {
+        const V3 acc = direct_row(s, idx[t], G, soft2);
+        ax[t] = acc.x;
+        ay[t] = acc.y;
+        az[t] = acc.z;
+    }
 }
 
 // Kick-drift-kick leapfrog with the direct force.

@@ -40,6 +40,7 @@ def lib():
             raise RuntimeError(f"oracle not built: {ORACLE_SO} (run __graft_entry__.build())")
         L = C.CDLL(ORACLE_SO)
         L.orc_direct.argtypes = [_i64, _d, _d, _D, _D, _D, _D, _D, _D, _D]
+        L.orc_direct_subset.argtypes = [_i64, _d, _d, _D, _D, _D, _D, _i64, _I64, _D, _D, _D]
         L.orc_leapfrog_direct.argtypes = [_i64, _d, _d, _d, _i, _i, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
         L.orc_rk4_direct.argtypes = [_i64, _d, _d, _d, _i, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
         L.orc_step1_direct.argtypes = [_i64, _i, _d, _d, _d, _i] + [_D] * 10
@@ -87,6 +88,16 @@ def direct(m, x, y, z, G=G_REF, soft2=0.0):
     n = len(m)
     ax, ay, az = (np.zeros(n) for _ in range(3))
     lib().orc_direct(n, G, soft2, m, x, y, z, ax, ay, az)
+    return ax, ay, az
+
+
+def direct_subset(m, x, y, z, idx, G=G_REF, soft2=0.0):
+    """Direct-sum accelerations of the targets idx only."""
+    m, x, y, z = map(_c, (m, x, y, z))
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    k = len(idx)
+    ax, ay, az = (np.zeros(k) for _ in range(3))
+    lib().orc_direct_subset(len(m), G, soft2, m, x, y, z, k, idx, ax, ay, az)
     return ax, ay, az
 
 

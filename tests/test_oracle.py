@@ -174,6 +174,18 @@ def test_step1_integrators_match_compiled_reference(O, golden, fixtures, integ, 
     assert sha256(*st) == golden[f"{integ}/{fx}/eps0.01/dt1e-3/steps{steps}"]["sha256"]
 
 
+def test_direct_subset_equals_full_rows(O, golden, fixtures):
+    """The target-subset direct sum reproduces the rows of the full sum, which
+    is bit-identical to the compiled reference."""
+    d = fixtures["p1000"]
+    full = O.direct(d["m"], d["x"], d["y"], d["z"], G_REF, 1e-4)
+    assert sha256(*full) == golden["direct/p1000/eps0.01"]["sha256"]
+    idx = np.array([999, 0, 500, 7, 7, 123])
+    sub = O.direct_subset(d["m"], d["x"], d["y"], d["z"], idx, G_REF, 1e-4)
+    for a, b in zip(sub, full):
+        assert np.array_equal(a, b[idx])
+
+
 def test_velocity_verlet_time_reversible(O, fixtures):
     d = fixtures["c1"]
     st = O.step1_direct("velocity_verlet", d["m"], d["x"], d["y"], d["z"], d["vx"], d["vy"],
