@@ -94,6 +94,43 @@ def stage_bytes(n, nodes):
     }
 
 
+def stage_report(stages, n, nodes):
+    """Per-stage ms per evaluation, with achieved GB/s and HBM fraction on the
+    algorithmic bytes of stage_bytes()."""
+    sb = stage_bytes(n, nodes)
+    out = {}
+    for k in ("bbox", "keys", "sort", "gather", "build", "upsweep", "walk"):
+        ms = stages[k] / stages["calls"]
+        e = {"ms": round(ms, 4)}
+        if k in sb:
+            gbs = sb[k] / (ms * 1e-3) / 1e9
+            e.update({"GB/s": round(gbs, 1), "hbm_frac": round(gbs / PEAK_HBM_GBS, 4),
+                      "bytes": int(sb[k])})
+        out[k] = e
+    return out
+
+
+def stages_at_scale(nb, n, evals=3):
+    """Tree stages of one force evaluation at a larger N (same Plummer model,
+    theta, eps, leaf capacity): at N = 2^20 the tree stages are bound by
+    kernel-launch latency, at 2^24 by memory traffic."""
+    pl = nb.ic_plummer(n, SEED)
+    f = nb.BarnesHutForceCalculator(EPS, THETA, CAP, "quadrupole", G=1.0)
+    p = nb.Particles.from_arrays(pl["m"], pl["x"], pl["y"], pl["z"])
+    del pl
+    f.computeAccelerations(p)
+    f.set_timing(True, stages=True)
+    f.reset_timing()
+    for _ in range(evals):
+        f.computeAccelerations(p)
+    stages = f.stage_timing()
+    f.set_timing(False)
+    out = {"n": n, "evals": evals, "stages": stage_report(stages, n, f.stats()["n_nodes"])}
+    p.free()
+    f.close()
+    return out
+
+
 def cpu_baseline(pl, n_sample):
     """Compiled reference (oracle/_ref) recursive Barnes-Hut on a bounded
     sample: full tree build + walk of n_sample evenly spaced targets +
@@ -321,6 +358,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workload", choices=("bh", "direct", "ensemble", "serve"), default="bh")
     ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--scale-n", type=int, default=1 << 24,
+                    help="also report the tree stages at this N (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -352,7 +391,7 @@ def main():
     log(f"[bench] warmup {W} steps")
     it.run(f, p, DT, W)
     nb.synchronize()
-    st_warm = f.stats()
+    st_warm = f.stats() if W > 0 else None
 
     # Timed steps record only three events per evaluation (start, walk start,
     # end): every event leaves the GPU idle for a few microseconds.
@@ -393,16 +432,7 @@ def main():
                 "flops_per_launch": int(flops),
                 "interactions_per_particle": {"pp": round(st["pp_interactions"] / n, 1),
                                               "pc": round(st["pc_interactions"] / n, 1)}}
-    sb = stage_bytes(n, st["n_nodes"])
-    stage_out = {}
-    for k in ("bbox", "keys", "sort", "gather", "build", "upsweep", "walk"):
-        ms = stages[k] / calls
-        e = {"ms": round(ms, 4)}
-        if k in sb:
-            gbs = sb[k] / (ms * 1e-3) / 1e9
-            e.update({"GB/s": round(gbs, 1), "hbm_frac": round(gbs / PEAK_HBM_GBS, 4),
-                      "bytes": int(sb[k])})
-        stage_out[k] = e
+    stage_out = stage_report(stages, n, st["n_nodes"])
 
     out = {
         "metric": "interactions/s (direct) & particle-steps/s (BH, N=1M); serve p50/p99 latency",
@@ -429,8 +459,12 @@ def main():
         "tree": {"nodes": st["n_nodes"], "leaves": st["n_leaves"], "groups": st["n_groups"],
                  "max_level": st["max_level"], "max_frontier": st["max_frontier"],
                  "frontier_overflow": st["frontier_overflow"], "oversize_cells": st["oversize_cells"],
-                 "nodes_warmup": st_warm["n_nodes"]},
+                 "nodes_warmup": st_warm["n_nodes"] if st_warm else None},
     }
+
+    if args.scale_n > 0:
+        log(f"[bench] tree stages at N={args.scale_n}")
+        out["stages_at_scale"] = stages_at_scale(nb, args.scale_n)
 
     if not args.no_accuracy:
         log("[bench] accuracy vs FP64 direct sum (all targets)")
