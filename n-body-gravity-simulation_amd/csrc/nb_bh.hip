@@ -48,7 +48,13 @@ constexpr int kSortRows = 16;
 constexpr int kSortTile = kBlk * kSortRows;
 constexpr int kSortBins = 256;
 constexpr int kSortPasses = 8;
-constexpr int kWalkFcap = 2048;  // walk frontier capacity per block (opened cells per level)
+// Walk frontier capacity per block (opened cells per level).  The frontier of
+// a group grows like theta^-3: 2048 entries at theta = 0.5 (the largest seen on
+// a 2^20 Plummer sphere is ~110), scaled for smaller theta up to kWalkFcapMax.
+// Only the part a group uses is touched, so the reservation costs HBM
+// capacity, not bandwidth.
+constexpr int kWalkFcap = 2048;
+constexpr int kWalkFcapMax = 65536;
 constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
 constexpr int kUpsweepGrid = 128;  // blocks per internal-level up-sweep launch
 constexpr int kGroup = 64;      // walk group: cells of at most 64 particles
@@ -1004,6 +1010,18 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
 }
 
 // ------------------------------------------------------------------ host
+int walk_fcap(double theta) {
+    if (!(theta < 0.5)) return kWalkFcap;
+    const double r = 0.5 / std::max(theta, 1e-3);
+    return (int)std::min<double>(kWalkFcapMax, std::ceil(kWalkFcap * r * r * r / 256.0) * 256.0);
+}
+
+int64_t walk_scratch_bytes(double theta, int cus) {
+    int bpc = kWalkBlocksPerCU;
+    if (const char* e = std::getenv("NB_WALK_BPC")) bpc = std::max(1, std::min(64, std::atoi(e)));
+    return (int64_t)std::max(1, cus) * bpc * 2 * walk_fcap(theta) * (int64_t)sizeof(int2);
+}
+
 int bh_create(BHWorkspace** w) {
     *w = new BHWorkspace();
     return NB_OK;
@@ -1145,14 +1163,15 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
         w->walk_blocks = std::max(1, cus) * bpc;
     }
     const int wblocks = std::min(ngroups, w->walk_blocks);
-    NB_TRY(w->walk_scratch.reserve((size_t)w->walk_blocks * 2 * kWalkFcap * sizeof(int2)));
+    const int fcap = walk_fcap(theta);
+    NB_TRY(w->walk_scratch.reserve((size_t)w->walk_blocks * 2 * fcap * sizeof(int2)));
     const float s2 = (float)soft2;
     const bool quad = multipole == NB_QUADRUPOLE;
 #define NB_WALK(Q, E)                                                                          \
     hipLaunchKernelGGL((k_walk<Q, E>), dim3(wblocks), dim3(64), 0, s, n, w->spos.as<float4>(), \
                        v0, w->w_mac.as<float4>(), w->w_link.as<int2>(), w->w_q0.as<float4>(),  \
                        w->w_q1.as<float4>(), s2, G, ax, ay, az,                                 \
-                       w->walk_scratch.as<int2>(), kWalkFcap, w->gstart.as<int>(), li, ctr)
+                       w->walk_scratch.as<int2>(), fcap, w->gstart.as<int>(), li, ctr)
     if (quad) {
         if (s2 > 0.f) NB_WALK(true, false);
         else NB_WALK(true, true);

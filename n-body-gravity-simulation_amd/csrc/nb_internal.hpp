@@ -70,6 +70,10 @@ int direct_eval(int precision, double G, double soft2, int64_t n, const double* 
 // ------------------------------------------------------------------ Barnes-Hut
 struct BHWorkspace;
 int bh_create(BHWorkspace** w);
+// Walk frontier entries per block per level for opening angle theta, and the
+// scratch bytes a calculator reserves for it on a GPU with `cus` CUs.
+int walk_fcap(double theta);
+int64_t walk_scratch_bytes(double theta, int cus);
 void bh_destroy(BHWorkspace* w);
 int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, int multipole,
             int64_t n, const double* m, const double* x, const double* y, const double* z,

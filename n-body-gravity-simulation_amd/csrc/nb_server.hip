@@ -165,11 +165,18 @@ struct Request {
     int done = 0;
 };
 
-int64_t request_bytes(int64_t n, bool bh, int leaf_cap) {
+int64_t request_bytes(int64_t n, bool bh, int leaf_cap, double theta) {
     // state (10 doubles) + packed FP32 copy; a BH request also needs its share
-    // of tree workspace beyond the lane's fixed part (node records ~ 3.3 n x 120 B).
+    // of tree workspace beyond the lane's fixed part (node records ~ 3.3 n x 120 B)
+    // and, below theta = 0.5, the larger walk frontier its lane will reserve.
     int64_t b = n * (80 + 16);
-    if (bh) b += n * (64 + 8) + (n + (int64_t)21 * (n / (leaf_cap + 1) + 1)) * 120;
+    if (bh) {
+        b += n * (64 + 8) + (n + (int64_t)21 * (n / (leaf_cap + 1) + 1)) * 120;
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        b += std::max<int64_t>(0, walk_scratch_bytes(theta, cus) - walk_scratch_bytes(0.5, cus));
+    }
     return b;
 }
 
@@ -346,7 +353,7 @@ int nb_server_submit(nb_server* s, const nb_request* req, int64_t* id) {
     auto r = std::make_unique<Request>();
     r->desc = *req;
     r->bh = req->n >= s->direct_threshold;
-    r->bytes = (size_t)request_bytes(req->n, r->bh, s->leaf_cap);
+    r->bytes = (size_t)request_bytes(req->n, r->bh, s->leaf_cap, req->theta);
     if ((double)r->bytes > s->budget) {
         set_error("nb_server_submit: request needs %zu bytes, budget is %.0f", r->bytes, s->budget);
         return NB_ERR_BUDGET;

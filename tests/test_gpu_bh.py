@@ -300,3 +300,24 @@ def test_plummer_16m_size_independent_properties(nb, O):
     e = rel_err(tuple(c[idx] for c in a), ref)
     assert np.percentile(e, 99) < 1e-3 and np.median(e) < 5e-4, (np.median(e), np.percentile(e, 99))
 
+
+def test_small_theta_large_n(nb, O):
+    """theta = 0.15 at N = 2^18: the walk frontier is sized from theta (about
+    40x more opened cells per level than at 0.5), nothing overflows, and the
+    sampled error is well below the theta = 0.5 error."""
+    n = 1 << 18
+    s = nb.ic_plummer(n, 11)
+    p = nb.Particles.from_arrays(s["m"], s["x"], s["y"], s["z"])
+    rng = np.random.default_rng(11)
+    idx = np.sort(rng.choice(n, 256, replace=False))
+    ref = O.direct_subset(s["m"], s["x"], s["y"], s["z"], idx, 1.0, 0.005 ** 2)
+    p99 = {}
+    for theta in (0.5, 0.15):
+        f = nb.BarnesHutForceCalculator(0.005, theta, 8, "quadrupole", G=1.0)
+        f.computeAccelerations(p)
+        st = f.stats()
+        assert st["frontier_overflow"] == 0, theta
+        e = rel_err(tuple(c[idx] for c in p.acc()), ref)
+        p99[theta] = np.percentile(e, 99)
+    assert p99[0.15] < p99[0.5] / 4, p99
+
