@@ -40,8 +40,7 @@ sys.path.insert(0, os.path.join(ROOT, "n-body-gravity-simulation_amd"))
 
 N_DEFAULT = 1 << 20
 THETA, EPS, DT, CAP, SEED = 0.5, 0.005, 1e-4, 8, 2
-PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector (packed) / dense FP32 MFMA
-PEAK_FP32_NONPACKED_TFLOPS = 78.6  # FP32 vector without packed instructions
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector / dense FP32 MFMA
 PEAK_HBM_GBS = 8000.0
 FLOP_PP, FLOP_PC_QUAD = 20, 60
 STAGE_EVALS = 3  # untimed evaluations for the per-stage breakdown
@@ -206,7 +205,6 @@ def bench_direct(args):
         "roofline": {"kernel": "k_direct_f32", "bound": "valu", "achieved": round(achieved, 2),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "frac_of_nonpacked_fp32": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 4),
                      "traffic": traffic_bytes("k_direct_f32"),
                      "traffic_source": traffic_source("k_direct_f32"),
                      "launch_ms": round(kern_ms, 4), "flops_per_launch": FLOP_PP * n * (n - 1)},
@@ -423,9 +421,7 @@ def main():
     flops = FLOP_PP * st["pp_interactions"] + FLOP_PC_QUAD * st["pc_interactions"]
     achieved = flops / (walk_ms * 1e-3) / 1e12
     roofline = {"kernel": "k_walk", "bound": "valu", "achieved": round(achieved, 2),
-                "peak_kind": "FP32 vector, packed (157.3 TFLOP/s is also the dense FP32 MFMA figure); "
-                             "this build runs non-packed FP32, whose peak is 78.6 TFLOP/s",
-                "frac_of_nonpacked_fp32": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 4),
+                "peak_kind": "FP32 vector (157.3 TFLOP/s is also the dense FP32 MFMA figure)",
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                 "traffic": traffic_bytes("k_walk"), "traffic_source": traffic_source("k_walk"),
                 "launch_ms": round(walk_ms, 4),
