@@ -56,7 +56,6 @@ constexpr int kSortPasses = 8;
 constexpr int kWalkFcap = 2048;
 constexpr int kWalkFcapMax = 65536;
 constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
-constexpr int kUpsweepGrid = 128;  // blocks per internal-level up-sweep launch
 constexpr int kGroup = 64;      // walk group: cells of at most 64 particles
 // Per-block count columns: 22 levels, group starts, leaf starts, oversize leaves.
 constexpr int kColGroup = kLevels, kColLeaf = kLevels + 1, kColOver = kLevels + 2;
@@ -1143,7 +1142,10 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->w_q0.as<float4>(), w->w_q1.as<float4>(), inv_theta2,
                        multipole == NB_QUADRUPOLE ? 1 : 0, n, ctr);
     for (int l = kLevels - 1; l >= 0; --l) {
-        hipLaunchKernelGGL(k_upsweep, dim3(kUpsweepGrid), dim3(kBlk), 0, s, l, li, root,
+        // One thread per node: a level has at most min(8^l, n) nodes.
+        const int64_t lvl_max = l < 8 ? std::min<int64_t>((int64_t)1 << (3 * l), n) : n;
+        const int ugrid = (int)std::max<int64_t>(1, (lvl_max + kBlk - 1) / kBlk);
+        hipLaunchKernelGGL(k_upsweep, dim3(ugrid), dim3(kBlk), 0, s, l, li, root,
                            w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
                            w->n_cend.as<int>(), w->n_level.as<signed char>(),
                            w->spos.as<float4>(), w->n_mom.as<double>(), w->w_mac.as<float4>(),
