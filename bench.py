@@ -124,7 +124,8 @@ def stages_at_scale(nb, n, evals=3):
         f.computeAccelerations(p)
     stages = f.stage_timing()
     f.set_timing(False)
-    out = {"n": n, "evals": evals, "stages": stage_report(stages, n, f.stats()["n_nodes"])}
+    nodes = f.stats()["n_nodes"]
+    out = {"n": n, "nodes": nodes, "evals": evals, "stages": stage_report(stages, n, nodes)}
     p.free()
     f.close()
     return out
