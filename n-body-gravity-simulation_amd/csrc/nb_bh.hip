@@ -739,8 +739,6 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
     if (ctr->overflow & 6) return;
     __shared__ int cl[128];
     __shared__ float4 pl[128];
-    __shared__ int eofs[64];
-    __shared__ int ebeg[64];
     __shared__ float4 st0[64], st1[64], st2[64];
     __shared__ int sgroup;
 
@@ -906,37 +904,28 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                 __syncthreads();
             }
 
+            // Opened leaves: each leaf's lane copies its own particles (a
+            // contiguous range of the sorted particles) into the staging list,
+            // 64 list slots per round.
             const int cnt = leaf ? (lk.y - ~lk.x) : 0;
             const int incl = wave_incl_scan(cnt);
             const int tot = __shfl(incl, 63, 64);
-            if (tot > 0) {
-                eofs[lane] = incl - cnt;
-                ebeg[lane] = leaf ? ~lk.x : 0;
+            const int first = leaf ? ~lk.x : 0;
+            for (int t0 = 0; t0 < tot; t0 += 64) {
+                // this lane's particles k with t0 <= excl + k < t0 + 64
+                const int lo = max(0, t0 - (incl - cnt)), hi = min(cnt, t0 + 64 - (incl - cnt));
+                for (int k = lo; k < hi; ++k) pl[npl + (incl - cnt) + k - t0] = spos[first + k];
+                npl += min(64, tot - t0);
                 __syncthreads();
-                for (int t0 = 0; t0 < tot; t0 += 64) {
-                    const int q = t0 + lane;
-                    if (q < tot) {
-                        int lo = 0, hi = 64;
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if (eofs[mid] <= q) lo = mid + 1;
-                            else hi = mid;
-                        }
-                        const int own = lo - 1;
-                        pl[npl + lane] = spos[ebeg[own] + (q - eofs[own])];
-                    }
-                    npl += min(64, tot - t0);
+                if (npl >= 64) {
+                    flush_particles(64);
+                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                    const bool mv = lane < npl - 64;
+                    if (mv) v = pl[64 + lane];
                     __syncthreads();
-                    if (npl >= 64) {
-                        flush_particles(64);
-                        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                        const bool mv = lane < npl - 64;
-                        if (mv) v = pl[64 + lane];
-                        __syncthreads();
-                        if (mv) pl[lane] = v;
-                        npl -= 64;
-                        __syncthreads();
-                    }
+                    if (mv) pl[lane] = v;
+                    npl -= 64;
+                    __syncthreads();
                 }
             }
         };
