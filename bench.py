@@ -285,7 +285,7 @@ def bench_serve(args):
         ns = np.rint(np.exp(rng.uniform(np.log(1e3), np.log(1e6), per_rate))).astype(int)
         steps = rng.integers(10, 1001, per_rate)
         systems = [nb.ic_plummer(int(n), 50000 + 100 * ri + k) for k, n in enumerate(ns)]
-        srv = nb.Server(hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=4)
+        srv = nb.Server(hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=args.bh_lanes)
         t0 = time.perf_counter()
         sub = 0
         done = {}
@@ -357,6 +357,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workload", choices=("bh", "direct", "ensemble", "serve"), default="bh")
     ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--bh-lanes", type=int, default=8,
+                    help="serve workload: Barnes-Hut calculators (streams) stepping concurrently")
     ap.add_argument("--scale-n", type=int, default=1 << 24,
                     help="also report the tree stages at this N (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",

@@ -521,7 +521,7 @@ class Ensemble:
 class Server:
     """Continuous-batching server (see nb_server_* in include/nbody_amd.h)."""
 
-    def __init__(self, hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=4,
+    def __init__(self, hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=8,
                  leaf_capacity=8):
         h = C.c_void_p()
         _check(lib.nb_server_create(C.byref(h), float(hbm_budget_bytes), int(direct_threshold),
