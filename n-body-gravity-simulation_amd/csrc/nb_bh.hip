@@ -998,6 +998,10 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
 }
 
 // ------------------------------------------------------------------ host
+static int64_t node_capacity(int64_t n, int cap) {
+    return n + (int64_t)kMaxDigits * ((n + cap) / (cap + 1)) + 1;
+}
+
 int walk_fcap(double theta) {
     if (!(theta < 0.5)) return kWalkFcap;
     const double r = 0.5 / std::max(theta, 1e-3);
@@ -1010,16 +1014,23 @@ int64_t walk_scratch_bytes(double theta, int cus) {
     return (int64_t)std::max(1, cus) * bpc * 2 * walk_fcap(theta) * (int64_t)sizeof(int2);
 }
 
+int64_t bh_workspace_bytes(int64_t n, int leaf_cap, double theta, int cus) {
+    if (n <= 0) return 0;
+    const int64_t nblk = (n + kBlk - 1) / kBlk, nsort = (n + kSortTile - 1) / kSortTile;
+    const int64_t ncap = node_capacity(n, leaf_cap);
+    const int64_t per_particle = 8 + 8 + 4 + 4 + 16 + 16 + 1 + 1 + 1 + 4 + 1 + 4;
+    const int64_t per_node = 4 * 4 + 1 + 10 * 8 + 16 + 8 + 16 + 16;
+    return n * per_particle + 4 + ncap * per_node + nsort * kSortBins * 4 + nblk * kCols * 4 +
+           (int64_t)std::max(kSortBins, kCols) * 4 + 1024 * 6 * 8 + 4096 +
+           walk_scratch_bytes(theta, cus);
+}
+
 int bh_create(BHWorkspace** w) {
     *w = new BHWorkspace();
     return NB_OK;
 }
 
 void bh_destroy(BHWorkspace* w) { delete w; }
-
-static int64_t node_capacity(int64_t n, int cap) {
-    return n + (int64_t)kMaxDigits * ((n + cap) / (cap + 1)) + 1;
-}
 
 int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, int multipole,
             int64_t n64, const double* m, const double* x, const double* y, const double* z,

@@ -74,6 +74,9 @@ int bh_create(BHWorkspace** w);
 // scratch bytes a calculator reserves for it on a GPU with `cus` CUs.
 int walk_fcap(double theta);
 int64_t walk_scratch_bytes(double theta, int cus);
+// Device bytes a Barnes-Hut calculator reserves for n particles (every
+// workspace buffer, walk scratch included).
+int64_t bh_workspace_bytes(int64_t n, int leaf_cap, double theta, int cus);
 void bh_destroy(BHWorkspace* w);
 int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, int multipole,
             int64_t n, const double* m, const double* x, const double* y, const double* z,

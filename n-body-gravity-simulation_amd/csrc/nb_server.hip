@@ -29,7 +29,6 @@
 namespace nb {
 
 constexpr int kSrvBlock = 256;
-constexpr int64_t kLaneBytes = (int64_t)256 << 20;
 
 struct SrvSys {
     const float4* pack;
@@ -165,20 +164,10 @@ struct Request {
     int done = 0;
 };
 
-int64_t request_bytes(int64_t n, bool bh, int leaf_cap, double theta) {
-    // state (10 doubles) + packed FP32 copy; a BH request also needs its share
-    // of tree workspace beyond the lane's fixed part (node records ~ 3.3 n x 120 B)
-    // and, below theta = 0.5, the larger walk frontier its lane will reserve.
-    int64_t b = n * (80 + 16);
-    if (bh) {
-        b += n * (64 + 8) + (n + (int64_t)21 * (n / (leaf_cap + 1) + 1)) * 120;
-        int dev = 0, cus = 256;
-        if (hipGetDevice(&dev) == hipSuccess)
-            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        b += std::max<int64_t>(0, walk_scratch_bytes(theta, cus) - walk_scratch_bytes(0.5, cus));
-    }
-    return b;
-}
+// Device bytes a request holds while it is active: its FP64 state (10
+// doubles per particle) and, for a direct request, the packed FP32 copy.  A
+// Barnes-Hut request's tree workspace lives in the lanes (see nb_server).
+int64_t request_bytes(int64_t n, bool bh) { return n * (bh ? 80 : 80 + 16); }
 
 }  // namespace
 
@@ -191,6 +180,13 @@ struct nb_server {
     std::vector<nb_force*> lanes;
     int64_t next_id = 1;
     int64_t bytes_used = 0;
+    // Barnes-Hut lanes run any BH request (round robin), and a calculator keeps
+    // its workspace at the largest size it has needed.  The budget therefore
+    // holds n_lanes workspaces sized for the largest n and smallest theta of
+    // any BH request admitted so far; that charge is never released.
+    int64_t lane_n = 0;
+    double lane_theta = 0.5;
+    int64_t lane_ws = 0;  // bytes charged per lane
     std::deque<std::unique_ptr<Request>> pending;
     std::vector<std::unique_ptr<Request>> active;
     std::map<int64_t, std::unique_ptr<Request>> done;
@@ -211,10 +207,21 @@ struct nb_server {
 
 namespace {
 
+// Extra budget needed to grow every lane to hold a BH request of n particles
+// at opening angle theta (0 if the lanes are already large enough).
+int64_t lane_growth(const nb_server* s, int64_t n, double theta, int64_t* ws_out) {
+    const int64_t ws = bh_workspace_bytes(std::max(n, s->lane_n), s->leaf_cap,
+                                          std::min(theta, s->lane_theta), s->cus);
+    if (ws_out) *ws_out = ws;
+    return (int64_t)s->n_lanes * std::max<int64_t>(0, ws - s->lane_ws);
+}
+
 int admit(nb_server* s) {
     while (!s->pending.empty()) {
         Request* r = s->pending.front().get();
-        if ((double)(s->bytes_used + (int64_t)r->bytes) > s->budget) break;  // wait for memory
+        int64_t ws = 0;
+        const int64_t grow = r->bh ? lane_growth(s, r->desc.n, r->desc.theta, &ws) : 0;
+        if ((double)(s->bytes_used + (int64_t)r->bytes + grow) > s->budget) break;  // wait for memory
         NB_TRY(nb_particles_alloc(&r->p, r->desc.n));
         const double* h[7];
         for (int k = 0; k < 7; ++k) h[k] = r->host[k].data();
@@ -222,6 +229,10 @@ int admit(nb_server* s) {
         for (auto& v : r->host) std::vector<double>().swap(v);
         if (r->bh) {
             NB_TRY(nb_integrator_create(&r->integ, NB_LEAPFROG));
+            s->bytes_used += grow;
+            s->lane_ws = std::max(s->lane_ws, ws);
+            s->lane_n = std::max(s->lane_n, r->desc.n);
+            s->lane_theta = std::min(s->lane_theta, r->desc.theta);
         } else {
             NB_TRY(r->pack.reserve((size_t)r->desc.n * sizeof(float4)));
         }
@@ -353,9 +364,15 @@ int nb_server_submit(nb_server* s, const nb_request* req, int64_t* id) {
     auto r = std::make_unique<Request>();
     r->desc = *req;
     r->bh = req->n >= s->direct_threshold;
-    r->bytes = (size_t)request_bytes(req->n, r->bh, s->leaf_cap, req->theta);
-    if ((double)r->bytes > s->budget) {
-        set_error("nb_server_submit: request needs %zu bytes, budget is %.0f", r->bytes, s->budget);
+    r->bytes = (size_t)request_bytes(req->n, r->bh);
+    // Alone on the server it still needs its state plus every lane's workspace.
+    const int64_t need = (int64_t)r->bytes +
+                         (r->bh ? (int64_t)s->n_lanes * bh_workspace_bytes(req->n, s->leaf_cap,
+                                                                           req->theta, s->cus)
+                                : 0);
+    if ((double)need > s->budget) {
+        set_error("nb_server_submit: request needs %lld bytes, budget is %.0f", (long long)need,
+                  s->budget);
         return NB_ERR_BUDGET;
     }
     const double* src[7] = {req->m, req->x, req->y, req->z, req->vx, req->vy, req->vz};
@@ -381,8 +398,7 @@ int nb_server_tick(nb_server* s, int* n_completed) {
                 nb_force* f = nullptr;
                 NB_TRY(nb_force_create_bh(&f, r->desc.G, r->desc.softening, r->desc.theta,
                                           s->leaf_cap, NB_QUADRUPOLE));
-                s->lanes.push_back(f);
-                s->bytes_used += kLaneBytes;  // the lane's walk scratch and fixed buffers
+                s->lanes.push_back(f);  // its workspace is charged at admission
             }
             nb_force* f = s->lanes[lane];
             // A lane is a workspace + stream; the physics parameters are the request's.

@@ -94,3 +94,44 @@ def test_budget_admission(nb):
     with pytest.raises(nb.NBError) as e:
         too_big.submit(sy[2], 1)
     assert e.value.code == 5  # NB_ERR_BUDGET
+
+
+def test_bh_lane_workspace_accounting(nb):
+    """Barnes-Hut lanes keep a workspace sized for the largest request they
+    have run: admission charges every lane's growth up front and never
+    releases it, while a request's own state is released when it finishes."""
+    n, steps = 3000, 3
+    sysd = nb.ic_plummer(n, 21)
+    state = n * 80
+    probe = nb.Server(hbm_budget_bytes=200e9, direct_threshold=1000, bh_lanes=2)
+    probe.submit(sysd, steps)
+    probe.tick()
+    lanes_plus_one = probe.status()["bytes_in_use"]
+    assert lanes_plus_one > state
+    while probe.status()["active"]:
+        probe.tick()
+    assert probe.status()["bytes_in_use"] == lanes_plus_one - state  # lanes keep theirs
+    alone = None
+    for rid in probe.poll():
+        alone = probe.result(rid)
+
+    # Room for the lanes and one request's state: the second request waits.
+    srv = nb.Server(hbm_budget_bytes=lanes_plus_one + state // 2, direct_threshold=1000,
+                    bh_lanes=2)
+    ids = [srv.submit(sysd, steps), srv.submit(sysd, steps)]
+    results, ticks = {}, 0
+    while len(results) < 2:
+        srv.tick()
+        st = srv.status()
+        assert st["active"] <= 1 and st["bytes_in_use"] <= lanes_plus_one
+        for rid in srv.poll():
+            results[rid] = srv.result(rid)
+        ticks += 1
+        assert ticks < 100
+    for rid in ids:
+        assert all(np.array_equal(results[rid][f], alone[f]) for f in FIELDS)
+
+    with pytest.raises(nb.NBError) as e:  # lanes alone exceed this budget
+        nb.Server(hbm_budget_bytes=state * 2, direct_threshold=1000, bh_lanes=2).submit(sysd, 1)
+    assert e.value.code == 5
+
