@@ -285,8 +285,12 @@ int nb_ensemble_tuning(const nb_ensemble* e, int32_t* shape_codes, int n_buckets
  * (n < direct_threshold) in one batched launch, Barnes-Hut requests on a pool
  * of bh_lanes calculators with their own streams, overlapping on the device.
  * Finished requests are downloaded and released; pending ones are admitted as
- * memory frees up.  No reference counterpart (the reference runs one system
- * per process). */
+ * memory frees up.  Budget accounting: every active request is charged its
+ * device state (80 B/particle; direct requests 16 B/particle more for the FP32
+ * copy); the Barnes-Hut lanes are charged their workspaces at the size of the
+ * largest n and smallest theta admitted so far, a charge that is never
+ * released (the lanes keep that memory).  No reference counterpart (the
+ * reference runs one system per process). */
 typedef struct nb_server nb_server;
 typedef struct nb_request {
     int64_t n;                  /* particles */
