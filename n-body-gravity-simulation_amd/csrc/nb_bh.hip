@@ -29,9 +29,13 @@
 //             size^2 < theta^2 * d^2 with d the distance from the cell's centre
 //             of mass to the group's bounding box (the reference criterion of
 //             force_calculator_barnes_hut_optimized.cpp:88, evaluated
-//             conservatively for every target of the group at once).  Accepted
-//             cells and opened leaves' particles are staged in LDS 64 at a time
-//             and evaluated by all lanes, two targets per lane.
+//             conservatively for every target of the group at once).  A node's
+//             walk record is read once per candidate; accepted cells go from
+//             registers to an LDS staging list, opened leaves' particles to a
+//             second one, and each list is evaluated by all lanes (two targets
+//             per lane) in batches of 60-64 entries, a multiple of the number
+//             of lane subsets.  The first 64 frontier entries of a level are
+//             kept in LDS.
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -737,13 +741,14 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                                              Counters* __restrict__ ctr) {
     // A tree that failed the build/up-sweep checks is not walked.
     if (ctr->overflow & 6) return;
-    __shared__ int cl[128];
-    __shared__ float4 pl[128];
-    __shared__ float4 st0[64], st1[64], st2[64];
+    __shared__ float4 pl[128];                 // staged source particles
+    __shared__ float4 sc0[64], sc1[64], sc2[64];  // staged cells: centre of mass, (M Qxx Qyy Qzz), (Qxy Qxz Qyz)
+    __shared__ int2 fl[2][64];                 // frontier entries 0..63 of this level / the next
     __shared__ int sgroup;
 
     const int lane = threadIdx.x;
     const unsigned long long lt = lanemask_lt();
+    // Frontier entries from 64 on live in a per-block slice of global scratch.
     int2* fr0 = scratch + (size_t)blockIdx.x * 2 * fcap;
     int2* fr1 = fr0 + fcap;
     const int ngroups = li->n_groups;
@@ -766,6 +771,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
         // sums are added in subset order at the end.
         const int nl = (nact + 1) >> 1;
         const int nsub = 64 / nl;
+        const int cfull = nsub * (64 / nsub);  // staged cells (particles) per evaluation batch, >= 60
         const int sub = lane / nl;
         const int tix = lane - sub * nl;
         const bool live = sub < nsub;
@@ -792,9 +798,9 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                     bh2 = fmaxf(hi2 - bc2, bc2 - lo2);
 
         float fxA = 0.f, fyA = 0.f, fzA = 0.f, fxB = 0.f, fyB = 0.f, fzB = 0.f;
-        int ncl = 0, npl = 0, nnext = 0;
+        int ncl = 0, npl = 0, nnext = 0, nb = 0;
         unsigned long long npp = 0, npc = 0;
-        int2* nxt = fr0;
+        int2* nxtg = fr0;
 
         auto pp_one = [&](const float4& s, const float4& me, float& fx, float& fy, float& fz) {
             const float dx = s.x - me.x, dy = s.y - me.y, dz = s.z - me.z;
@@ -844,18 +850,13 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             }
         };
 
+        // Evaluates the cnt staged cells; the staging slots are free afterwards.
         auto flush_cells = [&](int cnt) {
-            if (lane < cnt) {
-                const int id = cl[lane];
-                st0[lane] = mac[id];
-                st1[lane] = q0[id];
-                if (kQuad) st2[lane] = q1[id];
-            }
             __syncthreads();
             for (int j = live ? sub : cnt; j < cnt; j += nsub) {
-                const float4 c = st0[j];
-                const float4 qa = st1[j];
-                const float4 qb = kQuad ? st2[j] : qa;
+                const float4 c = sc0[j];
+                const float4 qa = sc1[j];
+                const float4 qb = kQuad ? sc2[j] : qa;
                 pc_one(c, qa, qb, meA, fxA, fyA, fzA);
                 pc_one(c, qa, qb, meB, fxB, fyB, fzB);
             }
@@ -863,15 +864,16 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             __syncthreads();
         };
 
-        // Classifies one candidate node per lane (id < 0: none) against the group.
+        // Classifies one candidate node per lane (id < 0: none) against the
+        // group.  The node's walk record is read once: an accepted cell goes
+        // from registers to the LDS staging list.
         auto process = [&](int id) {
             const bool valid = id >= 0;
-            float4 mc = make_float4(0.f, 0.f, 0.f, 0.f);
-            int2 lk = make_int2(0, 0);
-            if (valid) {
-                mc = mac[id];
-                lk = link[id];
-            }
+            const int idc = valid ? id : 0;  // lanes without a candidate read the root's record
+            const float4 mc = mac[idc];
+            const int2 lk = link[idc];
+            const float4 qa = q0[idc];
+            const float4 qb = kQuad ? q1[idc] : qa;
             const float e0 = fmaxf(fabsf(mc.x - bc0) - bh0, 0.f);
             const float e1 = fmaxf(fabsf(mc.y - bc1) - bh1, 0.f);
             const float e2 = fmaxf(fabsf(mc.z - bc2) - bh2, 0.f);
@@ -884,85 +886,125 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             const int no = __popcll(bo);
             if (open) {
                 const int pos = nnext + __popcll(bo & lt);
-                if (pos < fcap) nxt[pos] = lk;
+                if (pos < 64) fl[nb][pos] = lk;
+                else if (pos < fcap) nxtg[pos] = lk;
             }
             if (nnext + no > fcap) ovf = true;
             nnext = min(nnext + no, fcap);
 
+            // Accepted cells: staged in LDS and evaluated cfull at a time
+            // (cfull is a multiple of the subset count, so no subset idles).
+            // Cells past a full batch wait in registers during its evaluation.
             const unsigned long long ba = __ballot(acc);
-            if (acc) cl[ncl + __popcll(ba & lt)] = id;
-            ncl += __popcll(ba);
-            __syncthreads();
-            if (ncl >= 64) {
-                flush_cells(64);
-                const bool mv = lane < ncl - 64;
-                int v = 0;
-                if (mv) v = cl[64 + lane];
-                __syncthreads();
-                if (mv) cl[lane] = v;
-                ncl -= 64;
-                __syncthreads();
+            const int na = __popcll(ba);
+            const int pos = ncl + __popcll(ba & lt);  // < cfull + 63
+            if (acc && pos < cfull) {
+                sc0[pos] = mc;
+                sc1[pos] = qa;
+                if (kQuad) sc2[pos] = qb;
+            }
+            ncl += na;
+            if (ncl >= cfull) {
+                flush_cells(cfull);
+                if (acc && pos >= cfull) {
+                    sc0[pos - cfull] = mc;
+                    sc1[pos - cfull] = qa;
+                    if (kQuad) sc2[pos - cfull] = qb;
+                }
+                ncl -= cfull;
+                if (ncl >= cfull) {  // possible when cfull < 64: a second batch
+                    flush_cells(cfull);
+                    const bool mv = lane < ncl - cfull;
+                    float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0, v2 = v0;
+                    if (mv) {
+                        v0 = sc0[cfull + lane];
+                        v1 = sc1[cfull + lane];
+                        if (kQuad) v2 = sc2[cfull + lane];
+                    }
+                    __syncthreads();
+                    if (mv) {
+                        sc0[lane] = v0;
+                        sc1[lane] = v1;
+                        if (kQuad) sc2[lane] = v2;
+                    }
+                    ncl -= cfull;
+                }
             }
 
             // Opened leaves: each leaf's lane copies its own particles (a
             // contiguous range of the sorted particles) into the staging list,
-            // 64 list slots per round.
+            // 64 list slots per round, four loads in flight at a time.
             const int cnt = leaf ? (lk.y - ~lk.x) : 0;
             const int incl = wave_incl_scan(cnt);
             const int tot = __shfl(incl, 63, 64);
-            const int first = leaf ? ~lk.x : 0;
+            const int excl = incl - cnt;
+            const float4* src = spos + (leaf ? ~lk.x : 0);
             for (int t0 = 0; t0 < tot; t0 += 64) {
-                // this lane's particles k with t0 <= excl + k < t0 + 64
-                const int lo = max(0, t0 - (incl - cnt)), hi = min(cnt, t0 + 64 - (incl - cnt));
-                for (int k = lo; k < hi; ++k) pl[npl + (incl - cnt) + k - t0] = spos[first + k];
+                // this lane's particles k with t0 <= excl + k < t0 + 64, to
+                // slot npl + excl + k - t0 (npl < cfull <= 64 here, so the slot is < 128)
+                const int lo = max(0, t0 - excl), hi = min(cnt, t0 + 64 - excl);
+                for (int k = lo; k < hi; k += 4) {
+                    const float4 v0 = src[k];
+                    const float4 v1 = src[min(k + 1, hi - 1)];
+                    const float4 v2 = src[min(k + 2, hi - 1)];
+                    const float4 v3 = src[min(k + 3, hi - 1)];
+                    float4* dst = pl + (npl + excl + k - t0);
+                    dst[0] = v0;
+                    if (k + 1 < hi) dst[1] = v1;
+                    if (k + 2 < hi) dst[2] = v2;
+                    if (k + 3 < hi) dst[3] = v3;
+                }
                 npl += min(64, tot - t0);
                 __syncthreads();
-                if (npl >= 64) {
-                    flush_particles(64);
+                while (npl >= cfull) {  // npl < cfull + 64 <= 128
+                    flush_particles(cfull);
                     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                    const bool mv = lane < npl - 64;
-                    if (mv) v = pl[64 + lane];
+                    const bool mv = lane < npl - cfull;
+                    if (mv) v = pl[cfull + lane];
                     __syncthreads();
                     if (mv) pl[lane] = v;
-                    npl -= 64;
+                    npl -= cfull;
                     __syncthreads();
                 }
             }
         };
 
-        // Root (node 0), then one tree level per pass.
+        // Root (node 0), then one tree level per pass; the opened cells' child
+        // ranges of a level are its frontier (entries 0..63 in LDS buffer nb).
         process(lane == 0 ? 0 : -1);
-        int2* cur = fr0;
         int ncur = nnext;
+        __syncthreads();
         while (ncur > 0) {
             maxf = max(maxf, ncur);
-            nxt = cur == fr0 ? fr1 : fr0;
+            const int cb = nb;
+            const int2* curg = cb ? fr1 : fr0;
+            nb = cb ^ 1;
+            nxtg = nb ? fr1 : fr0;
             nnext = 0;
             for (int base = 0; base < ncur; base += 8) {
                 const int ps = base + (lane >> 3);
                 const int cs = lane & 7;
                 int id = -1;
                 if (ps < ncur) {
-                    const int2 rg = cur[ps];
+                    const int2 rg = ps < 64 ? fl[cb][ps] : curg[ps];
                     if (cs < rg.y - rg.x) id = rg.x + cs;
                 }
                 process(id);
             }
             __syncthreads();
-            cur = nxt;
             ncur = nnext;
         }
         if (ncl > 0) flush_cells(ncl);
         if (npl > 0) flush_particles(npl);
 
         if (nsub > 1) {  // reduce the subsets' partial sums in subset order
-            st0[lane] = make_float4(fxA, fyA, fzA, 0.f);
-            st1[lane] = make_float4(fxB, fyB, fzB, 0.f);
+            sc0[lane] = make_float4(fxA, fyA, fzA, 0.f);
+            sc1[lane] = make_float4(fxB, fyB, fzB, 0.f);
             __syncthreads();
             if (lead) {
                 for (int q = 1; q < nsub; ++q) {
-                    const float4 a = st0[q * nl + tix];
-                    const float4 b = st1[q * nl + tix];
+                    const float4 a = sc0[q * nl + tix];
+                    const float4 b = sc1[q * nl + tix];
                     fxA += a.x;
                     fyA += a.y;
                     fzA += a.z;
