@@ -7,7 +7,9 @@
 //             subdivision replayed per particle (child centre = c +/- (s/2)/2,
 //             octant = x>=c | y>=c<<1 | z>=c<<2; same file :159-161,:219-227).
 //             Sorting the keys orders particles by the CPU tree's child order.
-//   sort      LSD radix sort of (key, index), 8 passes of 8 bits, stable.
+//   sort      LSD radix sort of (key, index), 8 passes of 8 bits, stable: one
+//             digit-count kernel, then one kernel per pass (decoupled look-back
+//             across tiles, tile sorted in LDS, runs written contiguously).
 //   gather    sorted float4 (x, y, z, m).
 //   build     k_levels + k_col_scan + k_level_info + k_emit: the tree the reference's
 //             insertion build produces (a cell splits iff it holds more than
@@ -52,6 +54,8 @@ constexpr int kSortRows = 16;
 constexpr int kSortTile = kBlk * kSortRows;
 constexpr int kSortBins = 256;
 constexpr int kSortPasses = 8;
+constexpr int kSortHistBlocks = 512;   // grid cap of the digit-histogram kernel
+constexpr int kLookbackSpins = 1 << 22;  // polls before a tile gives up waiting (error bit 8)
 // Walk frontier capacity per block (opened cells per level).  The frontier of
 // a group grows like theta^-3: 2048 entries at theta = 0.5 (the largest seen on
 // a 2^20 Plummer sphere is ~110), scaled for smaller theta up to kWalkFcapMax.
@@ -84,11 +88,22 @@ struct LevelInfo {
 struct Counters {
     unsigned long long pp, pc;
     int max_frontier;
-    int overflow;   // bit 0: walk frontier, bit 1: node capacity, bit 2: malformed tree
+    int overflow;   // bit 0: walk frontier, bit 1: node capacity, bit 2: malformed tree,
+                    // bit 3: sort look-back wait limit
     int oversize;   // leaves at the key-depth limit with > cap particles
     int n_leaves;
     int next_group;  // work counter of the persistent walk
 };
+
+// Radix sort state: the digit counts of all passes (filled in one pass over
+// the keys) and each pass's tile counter.
+struct SortState {
+    int hist[kSortPasses][kSortBins];
+    int tile[kSortPasses];
+};
+
+// Errors that make the sorted order (and so every later stage) invalid.
+constexpr int kTreeBad = 2 | 4 | 8;
 
 }  // namespace nb
 
@@ -100,7 +115,7 @@ struct nb::BHWorkspace {
     DevBuf root, bbox_part, keys0, keys1, vals0, vals1, hist, spos, upos;
     DevBuf lvl_leaf, lvl_sh, blkcnt, blkoff, levels, leaf_of, counters;
     DevBuf n_pstart, n_pend, n_child, n_cend, n_level, n_mom, w_mac, w_link, w_q0, w_q1;
-    DevBuf walk_scratch, gflag, gstart, glev, totals;
+    DevBuf walk_scratch, gflag, gstart, glev, totals, sortst;
     int walk_blocks = 0;
     int group_size = 0;
 };
@@ -180,10 +195,14 @@ __global__ __launch_bounds__(kBlk) void k_bbox_partial(int n, const double* __re
 
 __global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* __restrict__ part,
                                                      RootInfo* __restrict__ root,
-                                                     Counters* __restrict__ ctr) {
+                                                     Counters* __restrict__ ctr,
+                                                     SortState* __restrict__ ss) {
     __shared__ double sh[6][kBlk];
     const int t = threadIdx.x;
     if (t == 0) reset_counters(ctr);
+#pragma unroll
+    for (int p = 0; p < kSortPasses; ++p) ss->hist[p][t] = 0;
+    if (t < kSortPasses) ss->tile[t] = 0;
     double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (int k = t; k < nparts; k += kBlk) {
 #pragma unroll
@@ -278,20 +297,31 @@ __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__
 }
 
 // ------------------------------------------------------------------ radix sort
-__global__ __launch_bounds__(kBlk) void k_radix_hist(int n, const unsigned long long* __restrict__ keys,
-                                                     int shift, int nblk, int* __restrict__ hist) {
-    __shared__ int h[kSortBins];
+// LSD radix sort of (key, index), 8 passes of 8 bits, stable, one kernel per
+// pass (single-pass "onesweep" scheme: digit counts of every pass are taken
+// once up front; each tile finds the keys of the same digit in earlier tiles
+// by decoupled look-back).
+//
+// k_sort_hist: the digit counts of all eight passes, and zeroes the tiles'
+// look-back words.
+__global__ __launch_bounds__(kBlk) void k_sort_hist(int n, const unsigned long long* __restrict__ keys,
+                                                    SortState* __restrict__ ss,
+                                                    unsigned* __restrict__ look, int nlook) {
+    __shared__ int h[kSortPasses][kSortBins];
     const int t = threadIdx.x;
-    h[t] = 0;
+#pragma unroll
+    for (int p = 0; p < kSortPasses; ++p) h[p][t] = 0;
+    for (int i = blockIdx.x * kBlk + t; i < nlook; i += gridDim.x * kBlk) look[i] = 0u;
     __syncthreads();
-    const int base = blockIdx.x * kSortTile;
-#pragma unroll 4
-    for (int r = 0; r < kSortRows; ++r) {
-        const int i = base + r * kBlk + t;
-        if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & 255ull)], 1);
+    for (int i = blockIdx.x * kBlk + t; i < n; i += gridDim.x * kBlk) {
+        const unsigned long long k = keys[i];
+#pragma unroll
+        for (int p = 0; p < kSortPasses; ++p) atomicAdd(&h[p][(int)((k >> (8 * p)) & 255ull)], 1);
     }
     __syncthreads();
-    hist[t * nblk + blockIdx.x] = h[t];
+#pragma unroll
+    for (int p = 0; p < kSortPasses; ++p)
+        if (h[p][t]) atomicAdd(&ss->hist[p][t], h[p][t]);
 }
 
 // Exclusive scan of every column of a column-major count table
@@ -323,38 +353,58 @@ __global__ __launch_bounds__(1024) void k_col_scan(int nrows, int* __restrict__ 
     if (t == 0) totals[blockIdx.x] = carry;
 }
 
-// Stable scatter: items of a tile are ranked in (row, wave, lane) order, which
-// is their input order.  Lanes holding the same digit are found with eight
-// ballots; per-wave digit counts go through LDS.
-__global__ __launch_bounds__(kBlk) void k_radix_scatter(
+// One pass: a block takes the next tile of kSortTile keys (tiles are handed
+// out in launch order, so every tile a block waits for belongs to a block that
+// is already running).  Wave w holds the w-th quarter of the tile in
+// registers and ranks its keys per digit in (row, lane) order without block
+// barriers; the per-wave counts give the tile's digit counts, which the
+// block publishes (flag 1) before adding up the counts of earlier tiles,
+// walking back until a tile with a published inclusive prefix (flag 2), and
+// publishing its own inclusive prefix.  The tile is then sorted by digit in
+// LDS (stable: wave, row, lane order is input order) and each digit's run is
+// written contiguously.  Look-back word: flag << 30 | count (counts < 2^30).
+__global__ __launch_bounds__(kBlk) void k_onesweep(
     int n, const unsigned long long* __restrict__ kin, const unsigned* __restrict__ vin,
-    unsigned long long* __restrict__ kout, unsigned* __restrict__ vout, int shift, int nblk,
-    const int* __restrict__ offs, const int* __restrict__ digit_tot) {
-    __shared__ int run[kSortBins];
-    __shared__ int wc[4][kSortBins];
+    unsigned long long* __restrict__ kout, unsigned* __restrict__ vout, int pass,
+    SortState* __restrict__ ss, unsigned* __restrict__ look, Counters* __restrict__ ctr) {
+    constexpr int kWaves = kBlk / 64;
+    constexpr int kPerWave = kSortTile / kWaves;  // 1024 keys, kSortRows rows of 64
+    __shared__ unsigned long long sk[kSortTile];  // keys sorted by digit, then values
+    __shared__ int cw[kWaves][kSortBins];  // per-wave digit counts, then per-wave slot starts
+    __shared__ int goff[kSortBins];        // output position of tile slot 0 of each digit
+    __shared__ int sred[2][kWaves];
+    __shared__ int stile, sbad;
     const int t = threadIdx.x;
     const int w = t >> 6;
-    {  // exclusive scan of the 256 digit totals -> start of each digit's bucket
-        const int v = digit_tot[t];
-        const int incl = wave_incl_scan(v);
-        if (lane_id() == 63) wc[0][w] = incl;
-        __syncthreads();
-        int before = 0;
-        for (int k = 0; k < w; ++k) before += wc[0][k];
-        run[t] = before + incl - v + offs[t * nblk + blockIdx.x];
-        __syncthreads();
+    const int lane = lane_id();
+    const int shift = 8 * pass;
+    if (t == 0) {
+        stile = atomicAdd(&ss->tile[pass], 1);
+        sbad = 0;
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) wc[k][t] = 0;
+    for (int k = 0; k < kWaves; ++k) cw[k][t] = 0;
     __syncthreads();
-    const unsigned long long lt = lanemask_lt();
-    const int base = blockIdx.x * kSortTile;
+    const int tile = stile;
+    const int base = tile * kSortTile;
+    const int tcnt = min(kSortTile, n - base);
+    const int wbase = base + w * kPerWave + lane;
+
+    unsigned long long key[kSortRows];
+    unsigned val[kSortRows];
+    int pos[kSortRows];
+#pragma unroll
     for (int r = 0; r < kSortRows; ++r) {
-        const int i = base + r * kBlk + t;
-        const bool valid = i < n;
-        const unsigned long long key = valid ? kin[i] : 0ull;
-        const unsigned val = valid ? vin[i] : 0u;
-        const int d = (int)((key >> shift) & 255ull);
+        const int i = wbase + r * 64;
+        key[r] = i < n ? kin[i] : 0ull;
+        val[r] = i < n ? vin[i] : 0u;
+    }
+    // Rank within the wave: lanes with the same digit found with eight ballots.
+    const unsigned long long lt = lanemask_lt();
+#pragma unroll
+    for (int r = 0; r < kSortRows; ++r) {
+        const bool valid = wbase + r * 64 < n;
+        const int d = (int)((key[r] >> shift) & 255ull);
         unsigned long long peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -362,20 +412,106 @@ __global__ __launch_bounds__(kBlk) void k_radix_scatter(
             peers &= ((d >> b) & 1) ? bb : ~bb;
         }
         const int rank = __popcll(peers & lt);
-        if (valid && rank == 0) wc[w][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int pos = run[d] + rank;
-            for (int k = 0; k < w; ++k) pos += wc[k][d];
-            kout[pos] = key;
-            vout[pos] = val;
-        }
-        __syncthreads();
-        run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) wc[k][t] = 0;
-        __syncthreads();
+        const int before = cw[w][d];
+        pos[r] = before + rank;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) cw[w][d] = before + __popcll(peers);
+        __builtin_amdgcn_wave_barrier();
     }
+    __syncthreads();
+
+    // Thread t owns digit t: tile count, publication, offsets.
+    int wc[kWaves];
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+        wc[k] = cw[k][t];
+        c += wc[k];
+    }
+    unsigned* my = look + (size_t)tile * kSortBins + t;
+    __hip_atomic_store(my, ((tile == 0 ? 2u : 1u) << 30) | (unsigned)c, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    const int g = ss->hist[pass][t];
+    const int ic = wave_incl_scan(c), ig = wave_incl_scan(g);
+    if (lane == 63) {
+        sred[0][w] = ic;
+        sred[1][w] = ig;
+    }
+    __syncthreads();
+    int loff = ic - c, dstart = ig - g;
+    for (int k = 0; k < w; ++k) {
+        loff += sred[0][k];
+        dstart += sred[1][k];
+    }
+
+    // Decoupled look-back for digit t.
+    int excl = 0;
+    if (tile > 0) {
+        int tt = tile - 1;
+        int spins = 0;
+        for (;;) {
+            const unsigned v = __hip_atomic_load(look + (size_t)tt * kSortBins + t, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned f = v >> 30;
+            if (f == 0u) {
+                if (++spins > kLookbackSpins) {
+                    atomicOr(&ctr->overflow, 8);
+                    sbad = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += (int)(v & 0x3fffffffu);
+            if (f == 2u || tt == 0) break;
+            --tt;
+        }
+        __hip_atomic_store(my, (2u << 30) | (unsigned)(excl + c), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    goff[t] = dstart + excl - loff;
+    {
+        int st = loff;  // slot of wave k's first key of digit t
+#pragma unroll
+        for (int k = 0; k < kWaves; ++k) {
+            cw[k][t] = st;
+            st += wc[k];
+        }
+    }
+    __syncthreads();
+
+    // Sort the tile's keys by digit in LDS; each digit's keys of this tile are
+    // one contiguous run in the output.  Then the same for the values, in the
+    // same LDS (the destination of slot j is remembered from the key pass).
+#pragma unroll
+    for (int r = 0; r < kSortRows; ++r) {
+        if (wbase + r * 64 < n) {
+            pos[r] += cw[w][(int)((key[r] >> shift) & 255ull)];
+            sk[pos[r]] = key[r];
+        }
+    }
+    __syncthreads();
+    if (sbad) return;  // offsets unknown: leave the output alone (the error bit is set)
+    int dst[kSortRows];
+#pragma unroll
+    for (int r = 0; r < kSortRows; ++r) {
+        const int j = t + r * kBlk;
+        dst[r] = -1;
+        if (j < tcnt) {
+            const unsigned long long k = sk[j];
+            dst[r] = goff[(int)((k >> shift) & 255ull)] + j;
+            if ((unsigned)dst[r] < (unsigned)n) kout[dst[r]] = k;
+        }
+    }
+    __syncthreads();
+    unsigned* sv = reinterpret_cast<unsigned*>(sk);
+#pragma unroll
+    for (int r = 0; r < kSortRows; ++r)
+        if (wbase + r * 64 < n) sv[pos[r]] = val[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortRows; ++r)
+        if ((unsigned)dst[r] < (unsigned)n) vout[dst[r]] = sv[t + r * kBlk];
 }
 
 __global__ __launch_bounds__(kBlk) void k_gather(int n, const unsigned* __restrict__ perm,
@@ -383,7 +519,8 @@ __global__ __launch_bounds__(kBlk) void k_gather(int n, const unsigned* __restri
                                                  float4* __restrict__ spos) {
     const int k = blockIdx.x * kBlk + threadIdx.x;
     if (k >= n) return;
-    spos[k] = upos[perm[k]];
+    const unsigned j = perm[k];
+    spos[k] = upos[j < (unsigned)n ? j : 0u];
 }
 
 // ------------------------------------------------------------------ build
@@ -692,7 +829,7 @@ __global__ __launch_bounds__(kBlk) void k_leaf_moments(
     const int* cend, const signed char* __restrict__ level, const float4* __restrict__ spos,
     double* mom, float4* mac, int2* link, float4* q0, float4* q1, double inv_theta2, int quad,
     int n, Counters* ctr) {
-    if (ctr->overflow & 6) return;
+    if (ctr->overflow & kTreeBad) return;
     const int total = li->total;
     for (int id = blockIdx.x * kBlk + threadIdx.x; id < total; id += gridDim.x * kBlk)
         if (child[id] < 0)
@@ -711,7 +848,7 @@ __global__ __launch_bounds__(kBlk) void k_upsweep(
     const int* cend, const signed char* __restrict__ level, const float4* __restrict__ spos,
     double* mom, float4* mac, int2* link, float4* q0, float4* q1, double inv_theta2, int quad,
     int n, Counters* ctr) {
-    if (ctr->overflow & 6) return;
+    if (ctr->overflow & kTreeBad) return;
     const int cnt = li->count[l];
     const int st = li->start[l];
     const int total = li->total;
@@ -740,7 +877,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                                              const LevelInfo* __restrict__ li,
                                              Counters* __restrict__ ctr) {
     // A tree that failed the build/up-sweep checks is not walked.
-    if (ctr->overflow & 6) return;
+    if (ctr->overflow & kTreeBad) return;
     __shared__ float4 pl[128];                 // staged source particles
     __shared__ float4 sc0[64], sc1[64], sc2[64];  // staged cells: centre of mass, (M Qxx Qyy Qzz), (Qxy Qxz Qyz)
     __shared__ int2 fl[2][64];                 // frontier entries 0..63 of this level / the next
@@ -1062,7 +1199,8 @@ int64_t bh_workspace_bytes(int64_t n, int leaf_cap, double theta, int cus) {
     const int64_t ncap = node_capacity(n, leaf_cap);
     const int64_t per_particle = 8 + 8 + 4 + 4 + 16 + 16 + 1 + 1 + 1 + 4 + 1 + 4;
     const int64_t per_node = 4 * 4 + 1 + 10 * 8 + 16 + 8 + 16 + 16;
-    return n * per_particle + 4 + ncap * per_node + nsort * kSortBins * 4 + nblk * kCols * 4 +
+    return n * per_particle + 4 + ncap * per_node + nsort * kSortBins * kSortPasses * 4 +
+           (int64_t)sizeof(SortState) + nblk * kCols * 4 +
            (int64_t)std::max(kSortBins, kCols) * 4 + 1024 * 6 * 8 + 4096 +
            walk_scratch_bytes(theta, cus);
 }
@@ -1100,7 +1238,8 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_TRY(w->keys1.reserve((size_t)n * 8));
     NB_TRY(w->vals0.reserve((size_t)n * 4));
     NB_TRY(w->vals1.reserve((size_t)n * 4));
-    NB_TRY(w->hist.reserve((size_t)nsort * kSortBins * 4));
+    NB_TRY(w->hist.reserve((size_t)nsort * kSortBins * kSortPasses * 4));  // look-back words
+    NB_TRY(w->sortst.reserve(sizeof(SortState)));
     NB_TRY(w->totals.reserve((size_t)std::max(kSortBins, kCols) * 4));
     NB_TRY(w->spos.reserve((size_t)n * 16));
     NB_TRY(w->upos.reserve((size_t)n * 16));
@@ -1133,20 +1272,20 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(kBlk), 0, s, n, x, y, z,
                        w->bbox_part.as<double>());
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlk), 0, s, nbb, w->bbox_part.as<double>(),
-                       root, ctr);
+                       root, ctr, w->sortst.as<SortState>());
     if (ev && ev[1]) NB_HIP(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(k_keys, dim3(nblk), dim3(kBlk), 0, s, n, x, y, z, m, root, k0, v0,
                        w->upos.as<float4>());
     NB_LAUNCH_CHECK();
     if (ev && ev[2]) NB_HIP(hipEventRecord(ev[2], s));
+    SortState* ss = w->sortst.as<SortState>();
+    unsigned* look = w->hist.as<unsigned>();
+    const int nlook = nsort * kSortBins * kSortPasses;
+    hipLaunchKernelGGL(k_sort_hist, dim3(std::min(nsort * kSortRows, kSortHistBlocks)), dim3(kBlk), 0,
+                       s, n, k0, ss, look, nlook);
     for (int p = 0; p < kSortPasses; ++p) {
-        const int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nsort), dim3(kBlk), 0, s, n, k0, shift, nsort,
-                           w->hist.as<int>());
-        hipLaunchKernelGGL(k_col_scan, dim3(kSortBins), dim3(1024), 0, s, nsort,
-                           w->hist.as<int>(), w->totals.as<int>());
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nsort), dim3(kBlk), 0, s, n, k0, v0, k1, v1,
-                           shift, nsort, w->hist.as<int>(), w->totals.as<int>());
+        hipLaunchKernelGGL(k_onesweep, dim3(nsort), dim3(kBlk), 0, s, n, k0, v0, k1, v1, p, ss,
+                           look + (size_t)p * nsort * kSortBins, ctr);
         std::swap(k0, k1);
         std::swap(v0, v1);
     }
@@ -1258,7 +1397,7 @@ int bh_stats(BHWorkspace* w, hipStream_t s, nb_bh_stats* out) {
     out->n_groups = li.n_groups;
     if (c.overflow) {
         set_error("Barnes-Hut: capacity exceeded or malformed tree (flags %d: 1=walk frontier, "
-                  "2=node pool, 4=tree consistency)",
+                  "2=node pool, 4=tree consistency, 8=sort look-back wait)",
                   c.overflow);
         return NB_ERR_CAPACITY;
     }
