@@ -208,7 +208,8 @@ def bench_direct(args):
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic_bytes("k_direct_f32"),
                      "traffic_source": traffic_source("k_direct_f32"),
-                     "launch_ms": round(kern_ms, 4), "flops_per_launch": FLOP_PP * n * (n - 1)},
+                     "launch_ms": round(kern_ms, 4), "flops_per_launch": FLOP_PP * n * (n - 1),
+                     "fma_probe": fma_probe(nb, achieved)},
         "accuracy": {"median_rel_vs_f64": float(np.median(err)),
                      "p99_rel_vs_f64": float(np.percentile(err, 99)),
                      "energy_drift_rel": float(abs(E1 - E0) / abs(E0))},
@@ -327,6 +328,16 @@ def bench_serve(args):
     }
 
 
+def fma_probe(nb, achieved):
+    """The FP32 ceiling without packed FP32 (the library is built without it):
+    v_fma_f32 issue rate with VGPR operands, measured in this run outside the
+    timed region, and the kernel's FLOP rate as a fraction of it."""
+    r = nb.probe_fp32_fma()
+    return {"tflops": round(r["tflops"], 2), "frac": round(achieved / r["tflops"], 4),
+            "what": "16 independent v_fma_f32 chains per lane, VGPR operands, 8 x 256-lane "
+                    "blocks per CU (nb_probe_fp32_fma)"}
+
+
 def copy_probe(nbytes=1 << 28):
     """A plain device-to-device copy of `nbytes` (HIP runtime copy kernel),
     used to calibrate the FETCH_SIZE / WRITE_SIZE counters."""
@@ -430,7 +441,8 @@ def main():
                 "launch_ms": round(walk_ms, 4),
                 "flops_per_launch": int(flops),
                 "interactions_per_particle": {"pp": round(st["pp_interactions"] / n, 1),
-                                              "pc": round(st["pc_interactions"] / n, 1)}}
+                                              "pc": round(st["pc_interactions"] / n, 1)},
+                "fma_probe": fma_probe(nb, achieved)}
     stage_out = stage_report(stages, n, st["n_nodes"])
 
     out = {

@@ -50,6 +50,13 @@ int nb_set_device(int device);
 int nb_synchronize(void);
 /* Library version string. */
 const char* nb_version(void);
+/* Diagnostic with no counterpart in the reference: FP32 issue-rate probe on the
+ * current device (16 independent v_fma_f32 chains per lane, VGPR operands, 8
+ * 256-lane blocks per CU).  The library is built without packed FP32, so this,
+ * not the published FP32 peak, is the arithmetic ceiling of its force kernels;
+ * the benchmark reports it beside their FLOP rates.  Synchronous; *ms (may be
+ * NULL) receives the timed launch's duration. */
+int nb_probe_fp32_fma(double* tflops, double* ms);
 
 /* ---------------------------------------------------------- particle state */
 /* Structure-of-arrays device state, FP64 throughout.  Replaces ParticleArrays
@@ -145,7 +152,8 @@ typedef struct nb_bh_stats {
     int64_t pp_interactions;   /* particle-particle interactions in the walk */
     int64_t pc_interactions;   /* particle-cell interactions in the walk */
     int32_t max_frontier;      /* largest walk frontier seen */
-    int32_t frontier_overflow; /* nonzero if the walk frontier overflowed (result invalid) */
+    int32_t frontier_overflow; /* error flags, nonzero = result invalid: 1 walk frontier full,
+                                  2 node pool full, 4 malformed tree, 8 sort look-back wait limit */
     double root_center[3];
     double root_size;
     int64_t n_groups;          /* walk groups (tree cells of <= 64 particles) */

@@ -87,6 +87,7 @@ def _load():
     sig = {
         "nb_last_error": ([], C.c_char_p),
         "nb_version": ([], C.c_char_p),
+        "nb_probe_fp32_fma": ([C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "nb_device_count": ([C.POINTER(C.c_int)], C.c_int),
         "nb_set_device": ([C.c_int], C.c_int),
         "nb_synchronize": ([], C.c_int),
@@ -166,6 +167,14 @@ def device_count() -> int:
     c = C.c_int(0)
     _check(lib.nb_device_count(C.byref(c)))
     return c.value
+
+
+def probe_fp32_fma() -> dict:
+    """FP32 FMA issue-rate probe (VGPR operands, no packed FP32): the
+    arithmetic ceiling of this library's force kernels on the current GPU."""
+    tf, ms = C.c_double(0.0), C.c_double(0.0)
+    _check(lib.nb_probe_fp32_fma(C.byref(tf), C.byref(ms)))
+    return {"tflops": tf.value, "ms": ms.value}
 
 
 def synchronize():

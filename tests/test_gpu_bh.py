@@ -148,6 +148,28 @@ def test_small_and_ragged(nb, O, n):
     assert np.array_equal(top["depth"], dep) and np.array_equal(top["path"], path)
 
 
+@pytest.mark.parametrize("n", [4096, 4097, 3 * 4096 + 17, 40000])
+def test_sort_stable_with_repeated_keys_across_tiles(nb, O, n):
+    """The radix sort runs one kernel per 8-bit pass over tiles of 4096 keys
+    chained by look-back.  Keys with many repeats (half the particles sit on 64
+    shared points) spread over one or more tiles, ragged last tile included,
+    must come out in the stable order: ties by original index, bit-exact
+    against numpy's stable argsort of the oracle keys."""
+    rng = np.random.default_rng(n)
+    pts = rng.uniform(-1.0, 1.0, size=(64, 3))
+    xyz = pts[rng.integers(0, 64, n)]
+    move = rng.random(n) < 0.5
+    xyz[move] += rng.uniform(-0.5, 0.5, size=(int(move.sum()), 3))
+    d = dict(m=np.ones(n), x=xyz[:, 0].copy(), y=xyz[:, 1].copy(), z=xyz[:, 2].copy())
+    f, p = run_bh(nb, d, eps=0.01, cap=8, G=1.0)
+    top = f.topology(n)
+    keys = O.keys63(d["x"], d["y"], d["z"])
+    assert np.array_equal(top["keys"], keys)
+    assert np.array_equal(top["perm"], np.argsort(keys, kind="stable").astype(np.uint32))
+    f.stats()  # raises on any error flag, the sort's look-back wait limit included
+    assert all(np.all(np.isfinite(v)) for v in p.acc())
+
+
 def test_identical_positions_refuse_to_split(nb, O):
     """All particles at one point: the root cube is 1e-9 wide, its half size is
     below the split threshold, so (as in the reference) one root leaf holds all."""

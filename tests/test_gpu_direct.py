@@ -264,3 +264,25 @@ def test_orbit_example_reproduces_reference_orbit_csv(nb, orbit_rows, tmp_path):
     w.close()
     assert (tmp_path / "py.csv").read_text().splitlines() == lines
 
+
+
+def test_fp32_fma_probe_bounds_the_fp32_kernel(nb):
+    """The issue-rate probe the benchmark reports beside the FP32 kernels:
+    positive, below the published packed-FP32 peak (157.3 TFLOP/s, which this
+    build cannot use), and not slower than the FP32 direct kernel's FLOP rate
+    on a mid-size input (the probe is the kernel's ceiling)."""
+    r = nb.probe_fp32_fma()
+    assert 0.0 < r["tflops"] < 157.3 and r["ms"] > 0.0
+    n = 65536
+    pl = nb.ic_plummer(n, 7)
+    f = nb.ForceCalculatorNaive(0.01, "f32", G=1.0)
+    p = nb.Particles.from_arrays(pl["m"], pl["x"], pl["y"], pl["z"])
+    f.computeAccelerations(p)
+    f.set_timing(True)
+    f.reset_timing()
+    for _ in range(3):
+        f.computeAccelerations(p)
+    t = f.timing()
+    ms = t["total_ms"] / t["calls"]
+    kernel_tflops = 20.0 * n * (n - 1) / (ms * 1e-3) / 1e12
+    assert kernel_tflops < 1.1 * r["tflops"]
