@@ -1071,6 +1071,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             // Opened leaves: each leaf's lane copies its own particles (a
             // contiguous range of the sorted particles) into the staging list,
             // 64 list slots per round, four loads in flight at a time.
+            if (__ballot(leaf) == 0ull) return;  // most candidate batches open no leaf
             const int cnt = leaf ? (lk.y - ~lk.x) : 0;
             const int incl = wave_incl_scan(cnt);
             const int tot = __shfl(incl, 63, 64);
