@@ -1124,7 +1124,15 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                 const int cs = lane & 7;
                 int id = -1;
                 if (ps < ncur) {
-                    const int2 rg = ps < 64 ? fl[cb][ps] : curg[ps];
+                    int2 rg;
+                    if (base < 64) {  // base is a multiple of 8: ps < 64
+                        rg = fl[cb][ps];
+                    } else {  // (an atomic load keeps the two reads from merging into a flat load)
+                        const unsigned long long v = __hip_atomic_load(
+                            reinterpret_cast<const unsigned long long*>(curg + ps), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                        rg = make_int2((int)(unsigned)v, (int)(unsigned)(v >> 32));
+                    }
                     if (cs < rg.y - rg.x) id = rg.x + cs;
                 }
                 process(id);
