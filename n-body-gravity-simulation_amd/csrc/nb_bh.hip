@@ -569,14 +569,30 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
         }
     }
     __syncthreads();
+    // Window maxima of sg over gsz+1 entries by doubling in place: after step
+    // k, sg[u] is the maximum of the original sg[u .. u + 2^k - 1] (entries
+    // whose range would pass the end keep a partial value and are never read).
+    const int ng = kBlk + gsz;
+    int kg = 0;
+    while ((2 << kg) <= gsz + 1) ++kg;
+    for (int k = 0; k < kg; ++k) {
+        const int h = 1 << k;
+        int v0 = 0, v1 = 0;
+        const int u1 = t + kBlk;
+        if (t + 2 * h <= ng) v0 = max((int)sg[t], (int)sg[t + h]);
+        if (u1 + 2 * h <= ng) v1 = max((int)sg[u1], (int)sg[u1 + h]);
+        __syncthreads();
+        if (t + 2 * h <= ng) sg[t] = (signed char)v0;
+        if (u1 + 2 * h <= ng) sg[u1] = (signed char)v1;
+        __syncthreads();
+    }
     const int i = b0 + t;
     const bool valid = i < n;
     const int lmax = root->lmax;
     int Li = -1, shi = 0, mx = -1, gl = -1;
     if (valid) {
         for (int q = 0; q <= cap; ++q) mx = max(mx, (int)ss[t + q]);
-        int mg = -1;
-        for (int q = 0; q <= gsz; ++q) mg = max(mg, (int)sg[t + q]);
+        const int mg = max((int)sg[t], (int)sg[t + gsz + 1 - (1 << kg)]);
         Li = min(lmax, mx + 1);
         gl = min(Li, mg + 1);
         shi = (i == 0) ? -1 : shared_levels(sk[H + t - 1], sk[H + t]);
@@ -629,7 +645,10 @@ __global__ void k_level_info(int n, const int* __restrict__ totals, LevelInfo* _
 }
 
 // Writes the node records: particle range, level, child range; and the leaf
-// of every sorted particle.
+// of every sorted particle.  Particle i starts the nodes of levels shi+1..Li,
+// ends those of levels shn+1..Li (shn: shared levels with i+1) and records
+// its leaf at level Li, so a wave only visits the levels from the smallest of
+// those bounds to its deepest leaf.
 __global__ __launch_bounds__(kBlk) void k_emit(
     int n, const signed char* __restrict__ Lin, const signed char* __restrict__ shin, int nblk,
     const int* __restrict__ blkoff, const LevelInfo* __restrict__ li, int node_cap,
@@ -637,30 +656,40 @@ __global__ __launch_bounds__(kBlk) void k_emit(
     int* __restrict__ cend, signed char* __restrict__ level, int* __restrict__ leaf_of,
     const unsigned char* __restrict__ gflag, int* __restrict__ gstart,
     Counters* __restrict__ ctr) {
-    __shared__ int wcnt[4][kCols];
+    __shared__ int wcnt[4][kCols];         // per-wave counts: nodes started per level, groups
+    __shared__ int lbase[kLevels];         // nodes of each level started by earlier blocks
     __shared__ int lstart[kLevels + 1];
     const int t = threadIdx.x;
     const int w = t >> 6;
+    const int lane = lane_id();
     const int i = blockIdx.x * kBlk + t;
     const bool valid = i < n;
     const int Li = valid ? (int)Lin[i] : -1;
-    const int shi = valid ? (int)shin[i] : 0;
+    const int shi = valid ? (int)shin[i] : kLevels;
     const int shn = (valid && i + 1 < n) ? (int)shin[i + 1] : -1;
+    if (lane < kCols) wcnt[w][lane] = 0;
     if (t <= kLevels) lstart[t] = li->start[t];
-    const unsigned long long lt = lanemask_lt();
-    int rex[kLevels];
+    if (t < kLevels) lbase[t] = blkoff[t * nblk + blockIdx.x];
+    // The wave's level range (lo, hi].
+    int lo = valid ? min(min(shi, shn), Li - 1) : kLevels, hi = Li;
 #pragma unroll
-    for (int l = 0; l < kLevels; ++l) {
+    for (int d = 1; d < 64; d <<= 1) {
+        lo = min(lo, __shfl_xor(lo, d, 64));
+        hi = max(hi, __shfl_xor(hi, d, 64));
+    }
+    lo = max(lo, -1);
+    const unsigned long long lt = lanemask_lt();
+    __builtin_amdgcn_wave_barrier();
+    for (int l = lo + 1; l <= hi; ++l) {
         const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
-        rex[l] = __popcll(bal & lt);
-        if (lane_id() == 0) wcnt[w][l] = __popcll(bal);
+        if (lane == 0) wcnt[w][l] = __popcll(bal);
     }
     const bool gs = valid && gflag[i];
     int grank;
     {
         const unsigned long long bal = __ballot(gs);
         grank = __popcll(bal & lt);
-        if (lane_id() == 0) wcnt[w][kLevels] = __popcll(bal);
+        if (lane == 0) wcnt[w][kLevels] = __popcll(bal);
     }
     __syncthreads();
     if (gs) {
@@ -668,35 +697,36 @@ __global__ __launch_bounds__(kBlk) void k_emit(
         for (int k = 0; k < w; ++k) base += wcnt[k][kLevels];
         gstart[base + grank] = i;
     }
-#pragma unroll
-    for (int l = 0; l < kLevels; ++l) {
-        int base = blkoff[l * nblk + blockIdx.x];
-        for (int k = 0; k < w; ++k) base += wcnt[k][l];
-        rex[l] += base;
-    }
-    if (!valid || li->total > node_cap) return;
-#pragma unroll
-    for (int l = 0; l < kLevels; ++l) {
-        if (l <= Li) {
+    if (li->total > node_cap) return;  // uniform: the pool check failed, nothing to write
+    // rank(l): nodes of level l started before this particle (all blocks).
+    auto rank = [&](int l) {
+        const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
+        int r = lbase[l] + __popcll(bal & lt);
+        for (int k = 0; k < w; ++k) r += wcnt[k][l];
+        return r;
+    };
+    int rl = lo + 1 >= 0 && lo + 1 < kLevels ? rank(lo + 1) : 0;
+    for (int l = max(lo + 1, 0); l <= hi; ++l) {
+        const int rn = l + 1 < kLevels ? rank(l + 1) : 0;  // the same at level l+1
+        if (valid && l <= Li && (l == Li || l > min(shi, shn))) {
             const bool starts = shi < l;
-            const int id = lstart[l] + rex[l] + (starts ? 1 : 0) - 1;
+            const int id = lstart[l] + rl + (starts ? 1 : 0) - 1;
             if (id < lstart[l] || id >= lstart[l + 1]) {
                 atomicOr(&ctr->overflow, 4);
-                continue;
+            } else {
+                if (starts) {
+                    pstart[id] = i;
+                    level[id] = (signed char)l;
+                    child[id] = (l < Li) ? lstart[l + 1] + rn : -1;
+                }
+                if (l > shn) {
+                    pend[id] = i + 1;
+                    cend[id] = (l < Li) ? lstart[l + 1] + rn + (shi <= l ? 1 : 0) : -1;
+                }
+                if (l == Li) leaf_of[i] = id;
             }
-            if (starts) {
-                pstart[id] = i;
-                level[id] = (signed char)l;
-                child[id] = (l < Li) ? lstart[l + 1] + rex[min(l + 1, kLevels - 1)] : -1;
-            }
-            if (l > shn) {
-                pend[id] = i + 1;
-                cend[id] = (l < Li)
-                               ? lstart[l + 1] + rex[min(l + 1, kLevels - 1)] + (shi <= l ? 1 : 0)
-                               : -1;
-            }
-            if (l == Li) leaf_of[i] = id;
         }
+        rl = rn;
     }
 }
 
@@ -1433,6 +1463,10 @@ int bh_topology(BHWorkspace* w, hipStream_t s, int64_t n, int32_t* depth, uint64
     NB_HIP(hipMemcpy(lv.data(), w->n_level.ptr, li.total, hipMemcpyDeviceToHost));
     for (int64_t q = 0; q < n; ++q) {
         const unsigned o = p[q];
+        if (o >= (unsigned)n || lf[q] < 0 || lf[q] >= li.total) {
+            set_error("Barnes-Hut topology: malformed tree (particle %lld)", (long long)q);
+            return NB_ERR_CAPACITY;
+        }
         const int L = lv[lf[q]];
         if (depth) depth[o] = L;
         if (path) path[o] = L == 0 ? 0ull : (k[q] >> (63 - 3 * L));
@@ -1465,6 +1499,10 @@ int bh_nodes(BHWorkspace* w, hipStream_t s, int64_t capacity, int64_t* count, in
     NB_HIP(hipMemcpy(mo.data(), w->n_mom.ptr, nn * 80, hipMemcpyDeviceToHost));
     NB_HIP(hipMemcpy(k.data(), w->keys0.ptr, w->last_n * 8, hipMemcpyDeviceToHost));
     for (int64_t q = 0; q < nn; ++q) {
+        if (ps[q] < 0 || ps[q] >= w->last_n) {
+            set_error("Barnes-Hut nodes: malformed tree (node %lld)", (long long)q);
+            return NB_ERR_CAPACITY;
+        }
         const int L = lv[q];
         if (level) level[q] = L;
         if (path) path[q] = L == 0 ? 0ull : (k[ps[q]] >> (63 - 3 * L));
