@@ -51,9 +51,10 @@ constexpr int kMaxDigits = 21;   // octant digits in a key
 constexpr int kLevels = 22;      // node levels 0..21
 constexpr int kBlk = 256;
 constexpr int kSortRows = 16;
-constexpr int kSortTile = kBlk * kSortRows;
 constexpr int kSortBins = 256;
 constexpr int kSortPasses = 8;
+constexpr int kOsThreads = 512;                  // threads of a radix-pass block
+constexpr int kOsTile = kOsThreads * kSortRows;  // keys per radix-pass tile
 constexpr int kSortHistBlocks = 512;   // grid cap of the digit-histogram kernel
 constexpr int kLookbackSpins = 1 << 22;  // polls before a tile gives up waiting (error bit 8)
 // Walk frontier capacity per block (opened cells per level).  The frontier of
@@ -353,9 +354,9 @@ __global__ __launch_bounds__(1024) void k_col_scan(int nrows, int* __restrict__ 
     if (t == 0) totals[blockIdx.x] = carry;
 }
 
-// One pass: a block takes the next tile of kSortTile keys (tiles are handed
+// One pass: a block takes the next tile of kOsTile keys (tiles are handed
 // out in launch order, so every tile a block waits for belongs to a block that
-// is already running).  Wave w holds the w-th quarter of the tile in
+// is already running).  Wave w holds the w-th part of the tile in
 // registers and ranks its keys per digit in (row, lane) order without block
 // barriers; the per-wave counts give the tile's digit counts, which the
 // block publishes (flag 1) before adding up the counts of earlier tiles,
@@ -363,16 +364,16 @@ __global__ __launch_bounds__(1024) void k_col_scan(int nrows, int* __restrict__ 
 // publishing its own inclusive prefix.  The tile is then sorted by digit in
 // LDS (stable: wave, row, lane order is input order) and each digit's run is
 // written contiguously.  Look-back word: flag << 30 | count (counts < 2^30).
-__global__ __launch_bounds__(kBlk) void k_onesweep(
+__global__ __launch_bounds__(kOsThreads) void k_onesweep(
     int n, const unsigned long long* __restrict__ kin, const unsigned* __restrict__ vin,
     unsigned long long* __restrict__ kout, unsigned* __restrict__ vout, int pass,
     SortState* __restrict__ ss, unsigned* __restrict__ look, Counters* __restrict__ ctr) {
-    constexpr int kWaves = kBlk / 64;
-    constexpr int kPerWave = kSortTile / kWaves;  // 1024 keys, kSortRows rows of 64
-    __shared__ unsigned long long sk[kSortTile];  // keys sorted by digit, then values
+    constexpr int kWaves = kOsThreads / 64;
+    constexpr int kPerWave = kOsTile / kWaves;  // kSortRows rows of 64 keys
+    __shared__ unsigned long long sk[kOsTile];  // keys sorted by digit, then values
     __shared__ int cw[kWaves][kSortBins];  // per-wave digit counts, then per-wave slot starts
     __shared__ int goff[kSortBins];        // output position of tile slot 0 of each digit
-    __shared__ int sred[2][kWaves];
+    __shared__ int sred[2][kSortBins / 64];
     __shared__ int stile, sbad;
     const int t = threadIdx.x;
     const int w = t >> 6;
@@ -382,12 +383,11 @@ __global__ __launch_bounds__(kBlk) void k_onesweep(
         stile = atomicAdd(&ss->tile[pass], 1);
         sbad = 0;
     }
-#pragma unroll
-    for (int k = 0; k < kWaves; ++k) cw[k][t] = 0;
+    for (int e = t; e < kWaves * kSortBins; e += kOsThreads) (&cw[0][0])[e] = 0;
     __syncthreads();
     const int tile = stile;
-    const int base = tile * kSortTile;
-    const int tcnt = min(kSortTile, n - base);
+    const int base = tile * kOsTile;
+    const int tcnt = min(kOsTile, n - base);
     const int wbase = base + w * kPerWave + lane;
 
     unsigned long long key[kSortRows];
@@ -420,57 +420,60 @@ __global__ __launch_bounds__(kBlk) void k_onesweep(
     }
     __syncthreads();
 
-    // Thread t owns digit t: tile count, publication, offsets.
+    // Thread t < 256 owns digit t: tile count, publication, offsets.
+    const bool owner = t < kSortBins;
     int wc[kWaves];
-    int c = 0;
+    int c = 0, ic = 0, ig = 0, g = 0;
+    unsigned* my = look + (size_t)tile * kSortBins + (owner ? t : 0);
+    if (owner) {
 #pragma unroll
-    for (int k = 0; k < kWaves; ++k) {
-        wc[k] = cw[k][t];
-        c += wc[k];
-    }
-    unsigned* my = look + (size_t)tile * kSortBins + t;
-    __hip_atomic_store(my, ((tile == 0 ? 2u : 1u) << 30) | (unsigned)c, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    const int g = ss->hist[pass][t];
-    const int ic = wave_incl_scan(c), ig = wave_incl_scan(g);
-    if (lane == 63) {
-        sred[0][w] = ic;
-        sred[1][w] = ig;
+        for (int k = 0; k < kWaves; ++k) {
+            wc[k] = cw[k][t];
+            c += wc[k];
+        }
+        __hip_atomic_store(my, ((tile == 0 ? 2u : 1u) << 30) | (unsigned)c, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        g = ss->hist[pass][t];
+        ic = wave_incl_scan(c);
+        ig = wave_incl_scan(g);
+        if (lane == 63) {
+            sred[0][w] = ic;
+            sred[1][w] = ig;
+        }
     }
     __syncthreads();
-    int loff = ic - c, dstart = ig - g;
-    for (int k = 0; k < w; ++k) {
-        loff += sred[0][k];
-        dstart += sred[1][k];
-    }
-
-    // Decoupled look-back for digit t.
-    int excl = 0;
-    if (tile > 0) {
-        int tt = tile - 1;
-        int spins = 0;
-        for (;;) {
-            const unsigned v = __hip_atomic_load(look + (size_t)tt * kSortBins + t, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned f = v >> 30;
-            if (f == 0u) {
-                if (++spins > kLookbackSpins) {
-                    atomicOr(&ctr->overflow, 8);
-                    sbad = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            excl += (int)(v & 0x3fffffffu);
-            if (f == 2u || tt == 0) break;
-            --tt;
+    if (owner) {
+        int loff = ic - c, dstart = ig - g;
+        for (int k = 0; k < w; ++k) {
+            loff += sred[0][k];
+            dstart += sred[1][k];
         }
-        __hip_atomic_store(my, (2u << 30) | (unsigned)(excl + c), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
-    goff[t] = dstart + excl - loff;
-    {
+        // Decoupled look-back for digit t.
+        int excl = 0;
+        if (tile > 0) {
+            int tt = tile - 1;
+            int spins = 0;
+            for (;;) {
+                const unsigned v = __hip_atomic_load(look + (size_t)tt * kSortBins + t,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned f = v >> 30;
+                if (f == 0u) {
+                    if (++spins > kLookbackSpins) {
+                        atomicOr(&ctr->overflow, 8);
+                        sbad = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (int)(v & 0x3fffffffu);
+                if (f == 2u || tt == 0) break;
+                --tt;
+            }
+            __hip_atomic_store(my, (2u << 30) | (unsigned)(excl + c), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        goff[t] = dstart + excl - loff;
         int st = loff;  // slot of wave k's first key of digit t
 #pragma unroll
         for (int k = 0; k < kWaves; ++k) {
@@ -495,7 +498,7 @@ __global__ __launch_bounds__(kBlk) void k_onesweep(
     int dst[kSortRows];
 #pragma unroll
     for (int r = 0; r < kSortRows; ++r) {
-        const int j = t + r * kBlk;
+        const int j = t + r * kOsThreads;
         dst[r] = -1;
         if (j < tcnt) {
             const unsigned long long k = sk[j];
@@ -511,7 +514,7 @@ __global__ __launch_bounds__(kBlk) void k_onesweep(
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSortRows; ++r)
-        if ((unsigned)dst[r] < (unsigned)n) vout[dst[r]] = sv[t + r * kBlk];
+        if ((unsigned)dst[r] < (unsigned)n) vout[dst[r]] = sv[t + r * kOsThreads];
 }
 
 __global__ __launch_bounds__(kBlk) void k_gather(int n, const unsigned* __restrict__ perm,
@@ -1234,7 +1237,7 @@ int64_t walk_scratch_bytes(double theta, int cus) {
 
 int64_t bh_workspace_bytes(int64_t n, int leaf_cap, double theta, int cus) {
     if (n <= 0) return 0;
-    const int64_t nblk = (n + kBlk - 1) / kBlk, nsort = (n + kSortTile - 1) / kSortTile;
+    const int64_t nblk = (n + kBlk - 1) / kBlk, nsort = (n + kOsTile - 1) / kOsTile;
     const int64_t ncap = node_capacity(n, leaf_cap);
     const int64_t per_particle = 8 + 8 + 4 + 4 + 16 + 16 + 1 + 1 + 1 + 4 + 1 + 4;
     const int64_t per_node = 4 * 4 + 1 + 10 * 8 + 16 + 8 + 16 + 16;
@@ -1261,7 +1264,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     }
     const int n = (int)n64;
     const int nblk = grid_for(n, kBlk);
-    const int nsort = grid_for(n, kSortTile);
+    const int nsort = grid_for(n, kOsTile);
     const int nbb = std::min(1024, nblk);
     const int64_t ncap = node_capacity(n, leaf_cap);
     if (ncap > INT_MAX / 2) {
@@ -1323,7 +1326,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     hipLaunchKernelGGL(k_sort_hist, dim3(std::min(nsort * kSortRows, kSortHistBlocks)), dim3(kBlk), 0,
                        s, n, k0, ss, look, nlook);
     for (int p = 0; p < kSortPasses; ++p) {
-        hipLaunchKernelGGL(k_onesweep, dim3(nsort), dim3(kBlk), 0, s, n, k0, v0, k1, v1, p, ss,
+        hipLaunchKernelGGL(k_onesweep, dim3(nsort), dim3(kOsThreads), 0, s, n, k0, v0, k1, v1, p, ss,
                            look + (size_t)p * nsort * kSortBins, ctr);
         std::swap(k0, k1);
         std::swap(v0, v1);
