@@ -117,7 +117,9 @@ int nb_force_destroy(nb_force* f);
 
 /* ForceCalculator::computeAccelerations(ParticleArrays&)
  * (CUDA/include/force_calculator.hpp:25-27; CPU form CPU/include/force_calculator.hpp:24-26).
- * Writes p->ax, p->ay, p->az. */
+ * Writes p->ax, p->ay, p->az.  Asynchronous: for Barnes-Hut, the capacity and
+ * tree-consistency checks run on the device, and nb_force_bh_stats() reports
+ * their outcome (an evaluation that failed one leaves its outputs invalid). */
 int nb_force_compute(nb_force* f, nb_particles* p);
 
 /* ForceCalculator::name() (CPU/include/force_calculator.hpp:28). */
