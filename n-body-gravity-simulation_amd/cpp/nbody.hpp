@@ -210,10 +210,14 @@ class ForceCalculator {
         mirror_.upload(particles);
         computeAccelerations(mirror_.d);
         mirror_.download(particles, false);
+        checkDevice();
     }
     virtual void computeAccelerations(ParticleArrays& particles) {
         nbody::check(nb_force_compute(h_, &particles), "computeAccelerations");
     }
+    // Throws if a check that ran on the device failed in the last evaluation
+    // (the synchronous std::vector calls run it after downloading).
+    virtual void checkDevice() const {}
     virtual std::string name() const { return nb_force_name(h_); }
     virtual const TimingStatistics& getTimeStatistics() const {
         sync_stats();
@@ -253,6 +257,14 @@ class BarnesHutForceCalculator : public ForceCalculator {
         nb_bh_stats s{};
         nbody::check(nb_force_bh_stats(h_, &s), "BarnesHutForceCalculator::stats");
         return s;
+    }
+    // Walk frontier, node pool, tree consistency and sort checks of the last
+    // evaluation (nb_force_bh_stats fails if any of them did; NB_ERR_STATE
+    // only means nothing was evaluated yet, e.g. an empty particle set).
+    void checkDevice() const override {
+        nb_bh_stats s{};
+        const int rc = nb_force_bh_stats(h_, &s);
+        if (rc != NB_ERR_STATE) nbody::check(rc, "Barnes-Hut device checks");
     }
     // CUDA/include/force_calculator_barnes_hut.hpp:120-125
     double getTotalComputeTime() const {
@@ -302,6 +314,7 @@ class TimeIntegrator {
         nb_integrator_reset(h_);  // host state may have changed since the last call
         step(calculator, mirror_.d, dt);
         mirror_.download(particles, true);
+        calculator.checkDevice();
     }
     virtual void step(ForceCalculator& calculator, ParticleArrays& particles, double dt) {
         nbody::check(nb_integrator_step(h_, calculator.handle(), &particles, dt), "step");
