@@ -94,6 +94,8 @@ struct Counters {
     int oversize;   // leaves at the key-depth limit with > cap particles
     int n_leaves;
     int next_group;  // work counter of the persistent walk
+    int gbucket[kGroup];  // walk groups per size bucket (bucket 0: >= kGroup particles)
+    int gcursor[kGroup];  // groups placed so far per bucket
 };
 
 // Radix sort state: the digit counts of all passes (filled in one pass over
@@ -116,7 +118,7 @@ struct nb::BHWorkspace {
     DevBuf root, bbox_part, keys0, keys1, vals0, vals1, hist, spos, upos;
     DevBuf lvl_leaf, lvl_sh, blkcnt, blkoff, levels, leaf_of, counters;
     DevBuf n_pstart, n_pend, n_child, n_cend, n_level, n_mom, w_mac, w_link, w_q0, w_q1;
-    DevBuf walk_scratch, gflag, gstart, glev, totals, sortst;
+    DevBuf walk_scratch, gflag, gstart, glev, totals, sortst, gorder;
     int walk_blocks = 0;
     int group_size = 0;
 };
@@ -201,6 +203,10 @@ __global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* _
     __shared__ double sh[6][kBlk];
     const int t = threadIdx.x;
     if (t == 0) reset_counters(ctr);
+    if (t < kGroup) {
+        ctr->gbucket[t] = 0;
+        ctr->gcursor[t] = 0;
+    }
 #pragma unroll
     for (int p = 0; p < kSortPasses; ++p) ss->hist[p][t] = 0;
     if (t < kSortPasses) ss->tile[t] = 0;
@@ -734,6 +740,57 @@ __global__ __launch_bounds__(kBlk) void k_emit(
 }
 
 
+// ------------------------------------------------------------------ walk order
+// The persistent walk hands out groups largest first, so the groups still
+// running when the queue empties are the cheapest ones (a group's cost grows
+// with its size: one interaction list evaluated for all of its targets).
+// Counting sort by size bucket, kGroup - min(size, kGroup); the order inside
+// a bucket is arbitrary (each group's result does not depend on it).
+__device__ __forceinline__ int group_bucket(const int* __restrict__ gstart, int g) {
+    return kGroup - min(gstart[g + 1] - gstart[g], kGroup);
+}
+
+__global__ __launch_bounds__(kBlk) void k_group_count(const LevelInfo* __restrict__ li,
+                                                      const int* __restrict__ gstart,
+                                                      Counters* __restrict__ ctr) {
+    __shared__ int h[kGroup];
+    const int t = threadIdx.x;
+    if (t < kGroup) h[t] = 0;
+    __syncthreads();
+    const int ng = li->n_groups;
+    for (int g = blockIdx.x * kBlk + t; g < ng; g += gridDim.x * kBlk)
+        atomicAdd(&h[group_bucket(gstart, g)], 1);
+    __syncthreads();
+    if (t < kGroup && h[t]) atomicAdd(&ctr->gbucket[t], h[t]);
+}
+
+__global__ __launch_bounds__(kBlk) void k_group_order(const LevelInfo* __restrict__ li,
+                                                      const int* __restrict__ gstart,
+                                                      Counters* __restrict__ ctr,
+                                                      int* __restrict__ order) {
+    __shared__ int bstart[kGroup], cnt[kGroup], base[kGroup];
+    const int t = threadIdx.x;
+    if (t == 0) {
+        int run = 0;
+        for (int b = 0; b < kGroup; ++b) {
+            bstart[b] = run;
+            run += ctr->gbucket[b];
+        }
+    }
+    const int ng = li->n_groups;
+    for (int g0 = blockIdx.x * kBlk; g0 < ng; g0 += gridDim.x * kBlk) {
+        if (t < kGroup) cnt[t] = 0;
+        __syncthreads();
+        const int g = g0 + t;
+        const int b = g < ng ? group_bucket(gstart, g) : 0;
+        const int r = g < ng ? atomicAdd(&cnt[b], 1) : 0;
+        __syncthreads();
+        if (t < kGroup && cnt[t]) base[t] = bstart[t] + atomicAdd(&ctr->gcursor[t], cnt[t]);
+        __syncthreads();
+        if (g < ng) order[base[b] + r] = g;
+    }
+}
+
 // ------------------------------------------------------------------ up-sweep
 // Moments of node `id` (FP64): a leaf from its particles, an internal node
 // from its (already finished) children in child order.  Writes the FP64
@@ -907,6 +964,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                                              double* __restrict__ ay, double* __restrict__ az,
                                              int2* __restrict__ scratch, int fcap,
                                              const int* __restrict__ gstart,
+                                             const int* __restrict__ gorder,
                                              const LevelInfo* __restrict__ li,
                                              Counters* __restrict__ ctr) {
     // A tree that failed the build/up-sweep checks is not walked.
@@ -927,7 +985,10 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
     bool ovf = false;
 
     for (;;) {
-        if (lane == 0) sgroup = atomicAdd(&ctr->next_group, 1);
+        if (lane == 0) {
+            const int k = atomicAdd(&ctr->next_group, 1);
+            sgroup = k < ngroups ? gorder[k] : ngroups;
+        }
         __syncthreads();
         const int grp = sgroup;
         __syncthreads();
@@ -1239,7 +1300,7 @@ int64_t bh_workspace_bytes(int64_t n, int leaf_cap, double theta, int cus) {
     if (n <= 0) return 0;
     const int64_t nblk = (n + kBlk - 1) / kBlk, nsort = (n + kOsTile - 1) / kOsTile;
     const int64_t ncap = node_capacity(n, leaf_cap);
-    const int64_t per_particle = 8 + 8 + 4 + 4 + 16 + 16 + 1 + 1 + 1 + 4 + 1 + 4;
+    const int64_t per_particle = 8 + 8 + 4 + 4 + 16 + 16 + 1 + 1 + 1 + 4 + 1 + 4 + 4;
     const int64_t per_node = 4 * 4 + 1 + 10 * 8 + 16 + 8 + 16 + 16;
     return n * per_particle + 4 + ncap * per_node + nsort * kSortBins * kSortPasses * 4 +
            (int64_t)sizeof(SortState) + nblk * kCols * 4 +
@@ -1290,6 +1351,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_TRY(w->blkcnt.reserve((size_t)nblk * kCols * 4));
     NB_TRY(w->gflag.reserve((size_t)n));
     NB_TRY(w->gstart.reserve((size_t)(n + 1) * 4));
+    NB_TRY(w->gorder.reserve((size_t)n * 4));
     NB_TRY(w->glev.reserve((size_t)n));
     NB_TRY(w->leaf_of.reserve((size_t)n * 4));
     NB_TRY(w->n_pstart.reserve((size_t)ncap * 4));
@@ -1355,6 +1417,13 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
                        w->n_cend.as<int>(), w->n_level.as<signed char>(), w->leaf_of.as<int>(),
                        w->gflag.as<unsigned char>(), w->gstart.as<int>(), ctr);
+    {
+        const int ggrid = std::min(grid_for(n, kBlk), 1024);
+        hipLaunchKernelGGL(k_group_count, dim3(ggrid), dim3(kBlk), 0, s, li, w->gstart.as<int>(),
+                           ctr);
+        hipLaunchKernelGGL(k_group_order, dim3(ggrid), dim3(kBlk), 0, s, li, w->gstart.as<int>(),
+                           ctr, w->gorder.as<int>());
+    }
     NB_LAUNCH_CHECK();
     if (ev && ev[5]) NB_HIP(hipEventRecord(ev[5], s));
     const double inv_theta2 = theta > 0.0 ? 1.0 / (theta * theta) : INFINITY;
@@ -1396,7 +1465,8 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     hipLaunchKernelGGL((k_walk<Q, E>), dim3(wblocks), dim3(64), 0, s, n, w->spos.as<float4>(), \
                        v0, w->w_mac.as<float4>(), w->w_link.as<int2>(), w->w_q0.as<float4>(),  \
                        w->w_q1.as<float4>(), s2, G, ax, ay, az,                                 \
-                       w->walk_scratch.as<int2>(), fcap, w->gstart.as<int>(), li, ctr)
+                       w->walk_scratch.as<int2>(), fcap, w->gstart.as<int>(),                \
+                       w->gorder.as<int>(), li, ctr)
     if (quad) {
         if (s2 > 0.f) NB_WALK(true, false);
         else NB_WALK(true, true);
