@@ -25,7 +25,8 @@
 //             traceless quadrupole in FP64 for every level; FP32 copies for
 //             the walk.
 //   walk      k_walk: persistent one-wave blocks take walk groups (the tree
-//             cells of at most 64 particles) from an atomic counter.  The wave
+//             cells of at most 64 particles) from an atomic counter, largest
+//             groups first (k_group_count + k_group_order).  The wave
 //             opens cells breadth-first (8 parents x 8 children per step, one
 //             child per lane), accepting a cell for the whole group when
 //             size^2 < theta^2 * d^2 with d the distance from the cell's centre
