@@ -1102,10 +1102,13 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
         auto process = [&](int id) {
             const bool valid = id >= 0;
             const int idc = valid ? id : 0;  // lanes without a candidate read the root's record
-            const float4 mc = mac[idc];
-            const int2 lk = link[idc];
-            const float4 qa = q0[idc];
-            const float4 qb = kQuad ? q1[idc] : qa;
+            // 32-bit byte offsets from the uniform array bases (node ids < 2^28).
+            const unsigned o16 = (unsigned)idc << 4, o8 = (unsigned)idc << 3;
+            const float4 mc = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(mac) + o16);
+            const int2 lk = *reinterpret_cast<const int2*>(reinterpret_cast<const char*>(link) + o8);
+            const float4 qa = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(q0) + o16);
+            const float4 qb =
+                kQuad ? *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(q1) + o16) : qa;
             const float e0 = fmaxf(fabsf(mc.x - bc0) - bh0, 0.f);
             const float e1 = fmaxf(fabsf(mc.y - bc1) - bh1, 0.f);
             const float e2 = fmaxf(fabsf(mc.z - bc2) - bh2, 0.f);
@@ -1171,16 +1174,21 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             const int incl = wave_incl_scan(cnt);
             const int tot = __shfl(incl, 63, 64);
             const int excl = incl - cnt;
-            const float4* src = spos + (leaf ? ~lk.x : 0);
+            const char* src = reinterpret_cast<const char*>(spos);  // + 32-bit byte offsets
+            const unsigned first16 = (unsigned)(leaf ? ~lk.x : 0) << 4;
             for (int t0 = 0; t0 < tot; t0 += 64) {
                 // this lane's particles k with t0 <= excl + k < t0 + 64, to
                 // slot npl + excl + k - t0 (npl < cfull <= 64 here, so the slot is < 128)
                 const int lo = max(0, t0 - excl), hi = min(cnt, t0 + 64 - excl);
                 for (int k = lo; k < hi; k += 4) {
-                    const float4 v0 = src[k];
-                    const float4 v1 = src[min(k + 1, hi - 1)];
-                    const float4 v2 = src[min(k + 2, hi - 1)];
-                    const float4 v3 = src[min(k + 3, hi - 1)];
+                    const unsigned o0 = first16 + ((unsigned)k << 4);
+                    const float4 v0 = *reinterpret_cast<const float4*>(src + o0);
+                    const float4 v1 = *reinterpret_cast<const float4*>(
+                        src + first16 + ((unsigned)min(k + 1, hi - 1) << 4));
+                    const float4 v2 = *reinterpret_cast<const float4*>(
+                        src + first16 + ((unsigned)min(k + 2, hi - 1) << 4));
+                    const float4 v3 = *reinterpret_cast<const float4*>(
+                        src + first16 + ((unsigned)min(k + 3, hi - 1) << 4));
                     float4* dst = pl + (npl + excl + k - t0);
                     dst[0] = v0;
                     if (k + 1 < hi) dst[1] = v1;
