@@ -25,8 +25,9 @@
 //             traceless quadrupole in FP64 for every level; FP32 copies for
 //             the walk.
 //   walk      k_walk: persistent one-wave blocks take walk groups (the tree
-//             cells of at most 64 particles) from an atomic counter, largest
-//             groups first (k_group_count + k_group_order).  The wave
+//             cells of at most 64 particles) from an atomic counter, in
+//             size classes largest first, key order inside a class (the order
+//             is written by k_emit).  The wave
 //             opens cells breadth-first (8 parents x 8 children per step, one
 //             child per lane), accepting a cell for the whole group when
 //             size^2 < theta^2 * d^2 with d the distance from the cell's centre
@@ -68,8 +69,13 @@ constexpr int kWalkFcapMax = 65536;
 constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
 constexpr int kGroup = 64;      // walk group: cells of at most 64 particles
 // Per-block count columns: 22 levels, group starts, leaf starts, oversize leaves.
+// Walk groups are handed out by size class, largest first (a group's cost
+// grows with its size: one interaction list evaluated for all its targets),
+// in key order inside a class: classes >= 48, 32-47, 16-31 and < 16 particles.
+constexpr int kGBuckets = 4;
 constexpr int kColGroup = kLevels, kColLeaf = kLevels + 1, kColOver = kLevels + 2;
-constexpr int kCols = kLevels + 3;
+constexpr int kColGB = kLevels + 3;  // kGBuckets columns: group starts per size class
+constexpr int kCols = kColGB + kGBuckets;
 
 struct RootInfo {
     double center[3];
@@ -85,6 +91,7 @@ struct LevelInfo {
     int total;
     int max_level;
     int n_groups;
+    int gb_start[kGBuckets];  // first walk-order slot of each group size class
 };
 
 struct Counters {
@@ -95,8 +102,6 @@ struct Counters {
     int oversize;   // leaves at the key-depth limit with > cap particles
     int n_leaves;
     int next_group;  // work counter of the persistent walk
-    int gbucket[kGroup];  // walk groups per size bucket (bucket 0: >= kGroup particles)
-    int gcursor[kGroup];  // groups placed so far per bucket
 };
 
 // Radix sort state: the digit counts of all passes (filled in one pass over
@@ -204,10 +209,6 @@ __global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* _
     __shared__ double sh[6][kBlk];
     const int t = threadIdx.x;
     if (t == 0) reset_counters(ctr);
-    if (t < kGroup) {
-        ctr->gbucket[t] = 0;
-        ctr->gcursor[t] = 0;
-    }
 #pragma unroll
     for (int p = 0; p < kSortPasses; ++p) ss->hist[p][t] = 0;
     if (t < kSortPasses) ss->tile[t] = 0;
@@ -612,7 +613,17 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
     }
     const bool leaf_start = valid && shi < Li;
     const bool group_start = valid && shi < gl;
-    if (valid) gflag[i] = group_start ? 1 : 0;
+    int gb = -1;  // size class of the group starting here
+    if (group_start) {
+        // the group is the level-gl cell holding i: i and the particles after
+        // it that share its first gl octant digits (counted up to kGroup)
+        int sz = 1;
+        while (sz < kGroup && i + sz < n && t + sz < kBlk + H &&
+               shared_levels(sk[H + t], sk[H + t + sz]) >= gl)
+            ++sz;
+        gb = sz >= 48 ? 0 : sz >= 32 ? 1 : sz >= 16 ? 2 : 3;
+    }
+    if (valid) gflag[i] = (unsigned char)(gb + 1);  // 0: no group starts here
     const bool over = leaf_start && mx + 1 > lmax && root->key_limited;
 #pragma unroll
     for (int l = 0; l < kLevels; ++l) {
@@ -626,6 +637,11 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
             wcnt[w][kColGroup] = __popcll(bg);
             wcnt[w][kColLeaf] = __popcll(bl);
             wcnt[w][kColOver] = __popcll(bo);
+        }
+#pragma unroll
+        for (int b = 0; b < kGBuckets; ++b) {
+            const unsigned long long bb = __ballot(gb == b);
+            if (lane_id() == 0) wcnt[w][kColGB + b] = __popcll(bb);
         }
     }
     __syncthreads();
@@ -648,6 +664,11 @@ __global__ void k_level_info(int n, const int* __restrict__ totals, LevelInfo* _
     li->total = run;
     li->max_level = ml;
     li->n_groups = totals[kColGroup];
+    int gs = 0;
+    for (int b = 0; b < kGBuckets; ++b) {
+        li->gb_start[b] = gs;
+        gs += totals[kColGB + b];
+    }
     gstart[totals[kColGroup]] = n;
     ctr->n_leaves = totals[kColLeaf];
     ctr->oversize = totals[kColOver];
@@ -664,7 +685,7 @@ __global__ __launch_bounds__(kBlk) void k_emit(
     const int* __restrict__ blkoff, const LevelInfo* __restrict__ li, int node_cap,
     int* __restrict__ pstart, int* __restrict__ pend, int* __restrict__ child,
     int* __restrict__ cend, signed char* __restrict__ level, int* __restrict__ leaf_of,
-    const unsigned char* __restrict__ gflag, int* __restrict__ gstart,
+    const unsigned char* __restrict__ gflag, int* __restrict__ gstart, int* __restrict__ gorder,
     Counters* __restrict__ ctr) {
     __shared__ int wcnt[4][kCols];         // per-wave counts: nodes started per level, groups
     __shared__ int lbase[kLevels];         // nodes of each level started by earlier blocks
@@ -694,18 +715,30 @@ __global__ __launch_bounds__(kBlk) void k_emit(
         const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
         if (lane == 0) wcnt[w][l] = __popcll(bal);
     }
-    const bool gs = valid && gflag[i];
-    int grank;
+    const int gf = valid ? (int)gflag[i] : 0;
+    const bool gs = gf != 0;
+    const int gb = gf - 1;  // size class of the group starting here
+    int grank, orank = 0;
     {
         const unsigned long long bal = __ballot(gs);
         grank = __popcll(bal & lt);
         if (lane == 0) wcnt[w][kLevels] = __popcll(bal);
+#pragma unroll
+        for (int b = 0; b < kGBuckets; ++b) {
+            const unsigned long long bb = __ballot(gb == b);
+            if (gb == b) orank = __popcll(bb & lt);
+            if (lane == 0) wcnt[w][kColGB + b] = __popcll(bb);
+        }
     }
     __syncthreads();
     if (gs) {
         int base = blkoff[kLevels * nblk + blockIdx.x];
         for (int k = 0; k < w; ++k) base += wcnt[k][kLevels];
         gstart[base + grank] = i;
+        // walk order: size classes largest first, key order inside a class
+        int o = li->gb_start[gb] + blkoff[(kColGB + gb) * nblk + blockIdx.x] + orank;
+        for (int k = 0; k < w; ++k) o += wcnt[k][kColGB + gb];
+        gorder[o] = base + grank;
     }
     if (li->total > node_cap) return;  // uniform: the pool check failed, nothing to write
     // rank(l): nodes of level l started before this particle (all blocks).
@@ -740,57 +773,6 @@ __global__ __launch_bounds__(kBlk) void k_emit(
     }
 }
 
-
-// ------------------------------------------------------------------ walk order
-// The persistent walk hands out groups largest first, so the groups still
-// running when the queue empties are the cheapest ones (a group's cost grows
-// with its size: one interaction list evaluated for all of its targets).
-// Counting sort by size bucket, kGroup - min(size, kGroup); the order inside
-// a bucket is arbitrary (each group's result does not depend on it).
-__device__ __forceinline__ int group_bucket(const int* __restrict__ gstart, int g) {
-    return kGroup - min(gstart[g + 1] - gstart[g], kGroup);
-}
-
-__global__ __launch_bounds__(kBlk) void k_group_count(const LevelInfo* __restrict__ li,
-                                                      const int* __restrict__ gstart,
-                                                      Counters* __restrict__ ctr) {
-    __shared__ int h[kGroup];
-    const int t = threadIdx.x;
-    if (t < kGroup) h[t] = 0;
-    __syncthreads();
-    const int ng = li->n_groups;
-    for (int g = blockIdx.x * kBlk + t; g < ng; g += gridDim.x * kBlk)
-        atomicAdd(&h[group_bucket(gstart, g)], 1);
-    __syncthreads();
-    if (t < kGroup && h[t]) atomicAdd(&ctr->gbucket[t], h[t]);
-}
-
-__global__ __launch_bounds__(kBlk) void k_group_order(const LevelInfo* __restrict__ li,
-                                                      const int* __restrict__ gstart,
-                                                      Counters* __restrict__ ctr,
-                                                      int* __restrict__ order) {
-    __shared__ int bstart[kGroup], cnt[kGroup], base[kGroup];
-    const int t = threadIdx.x;
-    if (t == 0) {
-        int run = 0;
-        for (int b = 0; b < kGroup; ++b) {
-            bstart[b] = run;
-            run += ctr->gbucket[b];
-        }
-    }
-    const int ng = li->n_groups;
-    for (int g0 = blockIdx.x * kBlk; g0 < ng; g0 += gridDim.x * kBlk) {
-        if (t < kGroup) cnt[t] = 0;
-        __syncthreads();
-        const int g = g0 + t;
-        const int b = g < ng ? group_bucket(gstart, g) : 0;
-        const int r = g < ng ? atomicAdd(&cnt[b], 1) : 0;
-        __syncthreads();
-        if (t < kGroup && cnt[t]) base[t] = bstart[t] + atomicAdd(&ctr->gcursor[t], cnt[t]);
-        __syncthreads();
-        if (g < ng) order[base[b] + r] = g;
-    }
-}
 
 // ------------------------------------------------------------------ up-sweep
 // Moments of node `id` (FP64): a leaf from its particles, an internal node
@@ -1425,14 +1407,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->lvl_sh.as<signed char>(), nblk, w->blkcnt.as<int>(), li, (int)ncap,
                        w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
                        w->n_cend.as<int>(), w->n_level.as<signed char>(), w->leaf_of.as<int>(),
-                       w->gflag.as<unsigned char>(), w->gstart.as<int>(), ctr);
-    {
-        const int ggrid = std::min(grid_for(n, kBlk), 1024);
-        hipLaunchKernelGGL(k_group_count, dim3(ggrid), dim3(kBlk), 0, s, li, w->gstart.as<int>(),
-                           ctr);
-        hipLaunchKernelGGL(k_group_order, dim3(ggrid), dim3(kBlk), 0, s, li, w->gstart.as<int>(),
-                           ctr, w->gorder.as<int>());
-    }
+                       w->gflag.as<unsigned char>(), w->gstart.as<int>(), w->gorder.as<int>(), ctr);
     NB_LAUNCH_CHECK();
     if (ev && ev[5]) NB_HIP(hipEventRecord(ev[5], s));
     const double inv_theta2 = theta > 0.0 ? 1.0 / (theta * theta) : INFINITY;
