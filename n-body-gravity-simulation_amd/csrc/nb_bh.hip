@@ -615,13 +615,13 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
     const bool group_start = valid && shi < gl;
     int gb = -1;  // size class of the group starting here
     if (group_start) {
-        // the group is the level-gl cell holding i: i and the particles after
-        // it that share its first gl octant digits (counted up to kGroup)
-        int sz = 1;
-        while (sz < kGroup && i + sz < n && t + sz < kBlk + H &&
-               shared_levels(sk[H + t], sk[H + t + sz]) >= gl)
-            ++sz;
-        gb = sz >= 48 ? 0 : sz >= 32 ? 1 : sz >= 16 ? 2 : 3;
+        // The group is the level-gl cell holding i: i and the particles after
+        // it sharing its first gl octant digits, so it has more than k
+        // particles iff particle i+k shares them.
+        auto longer = [&](int k) {
+            return i + k < n && shared_levels(sk[H + t], sk[H + t + k]) >= gl;
+        };
+        gb = longer(47) ? 0 : longer(31) ? 1 : longer(15) ? 2 : 3;
     }
     if (valid) gflag[i] = (unsigned char)(gb + 1);  // 0: no group starts here
     const bool over = leaf_start && mx + 1 > lmax && root->key_limited;
