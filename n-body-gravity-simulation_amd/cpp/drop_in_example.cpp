@@ -11,7 +11,9 @@
 //   3. the same state after 5 RK4Integrator steps through the CUDA-style
 //      ParticleArrays API;
 //   4. accelerations of BarnesHutForceCalculator(0.01, 0.5, 8) through the
-//      CUDA-style API.
+//      CUDA-style API;
+//   5. accelerations of the recursive-build ForceCalculatorBarnesHut(0.01, 0.5)
+//      through the CPU-style vector API.
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -105,6 +107,25 @@ int main(int argc, char** argv) {
             std::cout << bh.name() << " nodes=" << s.n_nodes << " build_ms=" << bh.getBuildTreeTime()
                       << " walk_ms=" << bh.getTraversalTime() << "\n";
             freeParticles(d);
+        }
+        // 5. CPU-style recursive Barnes-Hut (leaf capacity 1), vector API.
+        {
+            std::vector<Particle> ps = particles;
+            ForceCalculatorBarnesHut rec(0.01, 0.5);
+            rec.computeAccelerations(ps);
+            std::vector<double> ax(N), ay(N), az(N);
+            for (int i = 0; i < N; ++i) {
+                ax[i] = ps[i].acceleration.x;
+                ay[i] = ps[i].acceleration.y;
+                az[i] = ps[i].acceleration.z;
+            }
+            put(out, ax);
+            put(out, ay);
+            put(out, az);
+            std::cout << rec.name() << " (recursive) build_ms="
+                      << rec.getBuildingTreeStatistics().total_time
+                      << " walk_ms=" << rec.getTraversalStatistics().total_time
+                      << " calls=" << rec.getTraversalStatistics().calls << "\n";
         }
         // Error path: an invalid theta is reported as an exception, not exit().
         try {

@@ -294,6 +294,42 @@ class ForceCalculatorBarnesHutOptimized : public BarnesHutForceCalculator {
         : BarnesHutForceCalculator(softening, theta, leaf_capacity) {}
 };
 
+// ForceCalculatorBarnesHut() / (double softening, double theta):
+// CPU/include/force_calculator_barnes_hut.hpp:33-34, the recursive build
+// (CPU/src/force_calculator_barnes_hut.cpp:97-173): a leaf splits when a second
+// particle arrives unless its half size is below 1e-9, i.e. the same tree
+// rule with leaf capacity 1.  The three statistics getters of :51-59 report
+// the GPU tree build, the walk, and no destruction (the device tree is reused).
+class ForceCalculatorBarnesHut : public BarnesHutForceCalculator {
+    mutable TimingStatistics building_{"Barnes-Hut Tree Building"}, traversal_{"Barnes-Hut Tree Traversal"},
+        destroying_{"Barnes-Hut Tree Destruction"};
+    void sync_parts() const {
+        double t[4];
+        if (nb_force_timing(h_, t) == NB_OK) {
+            building_.total_time = t[1];
+            traversal_.total_time = t[2];
+            building_.calls = traversal_.calls = destroying_.calls = (size_t)t[3];
+        }
+    }
+
+   public:
+    ForceCalculatorBarnesHut() : ForceCalculatorBarnesHut(0.0, 0.5) {}
+    ForceCalculatorBarnesHut(double softening, double theta)
+        : BarnesHutForceCalculator(softening, theta, 1) {}
+    TimingStatistics& getBuildingTreeStatistics() const {
+        sync_parts();
+        return building_;
+    }
+    TimingStatistics& getTraversalStatistics() const {
+        sync_parts();
+        return traversal_;
+    }
+    TimingStatistics& getDestroyingTreeStatistics() const {
+        sync_parts();
+        return destroying_;
+    }
+};
+
 // TimeIntegrator base with both step() signatures
 // (CPU/include/integrator.hpp:16-20, CUDA/include/integrator.hpp:10-14).
 class TimeIntegrator {
