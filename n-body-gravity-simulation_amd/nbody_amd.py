@@ -337,6 +337,23 @@ class BarnesHutForceCalculator(ForceCalculator):
         return dict(level=level, path=path, first=first, last=last, mass=mass, com=com, quad=quad)
 
 
+class ForceCalculatorBarnesHutOptimized(BarnesHutForceCalculator):
+    """ForceCalculatorBarnesHutOptimized(softening, theta, leaf_capacity)
+    (CPU/include/force_calculator_barnes_hut_optimized.hpp:40)."""
+
+    def __init__(self, softening: float, theta: float, leaf_capacity: int, G: float = G_REF):
+        super().__init__(softening, theta, leaf_capacity, "quadrupole", G)
+
+
+class ForceCalculatorBarnesHut(BarnesHutForceCalculator):
+    """ForceCalculatorBarnesHut(softening, theta) (CPU/include/force_calculator_barnes_hut.hpp:33-34):
+    the recursive build splits a leaf when a second particle arrives (unless its
+    half size is below 1e-9), i.e. the same tree rule with leaf capacity 1."""
+
+    def __init__(self, softening: float = 0.0, theta: float = 0.5, G: float = G_REF):
+        super().__init__(softening, theta, 1, "quadrupole", G)
+
+
 # --------------------------------------------------------------------------- integrators
 class TimeIntegrator:
     _kind = NB_LEAPFROG

@@ -130,6 +130,26 @@ def test_theta_zero_is_direct_sum(nb, O):
     assert e.max() < 1e-5
 
 
+def test_recursive_class_is_leaf_capacity_one(nb, O, fixtures):
+    """ForceCalculatorBarnesHut(softening, theta) mirrors the reference's
+    recursive-build class: its tree is the reference tree with one particle
+    per leaf (leaf depths and octant paths bit-exact against the oracle, itself
+    pinned to the compiled reference), and its forces are at least as accurate
+    as the reference recursive walk's on the same input (p99 vs FP64 direct)."""
+    d = fixtures["p1000"]
+    f = nb.ForceCalculatorBarnesHut(0.01, 0.5)
+    p = nb.Particles.from_arrays(d["m"], d["x"], d["y"], d["z"])
+    f.computeAccelerations(p)
+    n = len(d["m"])
+    top = f.topology(n)
+    dep, path, _, _ = O.bhopt_leaves(d["m"], d["x"], d["y"], d["z"], 1)
+    assert np.array_equal(top["depth"], dep) and np.array_equal(top["path"], path)
+    ref = O.direct(d["m"], d["x"], d["y"], d["z"], G_REF, 1e-4)
+    e_gpu = rel_err(p.acc(), ref)
+    e_rec = rel_err(O.bhrec(d["m"], d["x"], d["y"], d["z"], 0.5, G_REF, 1e-4), ref)
+    assert np.percentile(e_gpu, 99) <= np.percentile(e_rec, 99)
+
+
 @pytest.mark.parametrize("n", [1, 2, 5, 8, 9, 63, 64, 65, 1000])
 def test_small_and_ragged(nb, O, n):
     rng = np.random.default_rng(n + 100)
