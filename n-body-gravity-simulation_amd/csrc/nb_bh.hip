@@ -52,13 +52,18 @@ namespace nb {
 constexpr int kMaxDigits = 21;   // octant digits in a key
 constexpr int kLevels = 22;      // node levels 0..21
 constexpr int kBlk = 256;
-constexpr int kSortRows = 16;
 constexpr int kSortBins = 256;
 constexpr int kSortPasses = 8;
+constexpr int kSortRows = 16;                    // rows of 64 keys per wave in a radix-pass tile
 constexpr int kOsThreads = 512;                  // threads of a radix-pass block
 constexpr int kOsTile = kOsThreads * kSortRows;  // keys per radix-pass tile
-constexpr int kSortHistBlocks = 512;   // grid cap of the digit-histogram kernel
+constexpr int kSortHistKeys = 8;       // keys per thread per batch of the digit-histogram kernel
+constexpr int kSortHistBlocks = 1024;  // grid cap of the digit-histogram kernel
+constexpr int kSortHistCopies = 8;     // copies of the digit table per histogram block
+constexpr int kMomBatch = 8;  // particles per batch of loads in a leaf's moments
+constexpr int kUpsweepBlocks = 2048;  // grid cap of a per-level up-sweep launch
 constexpr int kLookbackSpins = 1 << 22;  // polls before a tile gives up waiting (error bit 8)
+constexpr int kLookWindow = 8;  // earlier tiles read per look-back round trip
 // Walk frontier capacity per block (opened cells per level).  The frontier of
 // a group grows like theta^-3: 2048 entries at theta = 0.5 (the largest seen on
 // a 2^20 Plummer sphere is ~110), scaled for smaller theta up to kWalkFcapMax.
@@ -312,25 +317,46 @@ __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__
 // by decoupled look-back).
 //
 // k_sort_hist: the digit counts of all eight passes, and zeroes the tiles'
-// look-back words.
+// look-back words.  Each thread loads kSortHistKeys keys before counting, so
+// a batch costs one memory round trip.  The high digits of clustered inputs
+// take few values, so lanes count into kSortHistCopies copies of the table
+// (lane mod kSortHistCopies; rows padded by one word so that equal digits of
+// different copies sit in different LDS banks) to cut same-address atomics.
 __global__ __launch_bounds__(kBlk) void k_sort_hist(int n, const unsigned long long* __restrict__ keys,
                                                     SortState* __restrict__ ss,
                                                     unsigned* __restrict__ look, int nlook) {
-    __shared__ int h[kSortPasses][kSortBins];
+    constexpr int kRow = kSortBins + 1;
+    __shared__ int h[kSortHistCopies * kSortPasses * kRow];
     const int t = threadIdx.x;
-#pragma unroll
-    for (int p = 0; p < kSortPasses; ++p) h[p][t] = 0;
+    for (int e = t; e < kSortHistCopies * kSortPasses * kRow; e += kBlk) h[e] = 0;
     for (int i = blockIdx.x * kBlk + t; i < nlook; i += gridDim.x * kBlk) look[i] = 0u;
     __syncthreads();
-    for (int i = blockIdx.x * kBlk + t; i < n; i += gridDim.x * kBlk) {
-        const unsigned long long k = keys[i];
+    int* hc = h + (t % kSortHistCopies) * kSortPasses * kRow;
+    constexpr int kBatch = kBlk * kSortHistKeys;
+    for (int b0 = blockIdx.x * kBatch; b0 < n; b0 += gridDim.x * kBatch) {
+        unsigned long long k[kSortHistKeys];
 #pragma unroll
-        for (int p = 0; p < kSortPasses; ++p) atomicAdd(&h[p][(int)((k >> (8 * p)) & 255ull)], 1);
+        for (int r = 0; r < kSortHistKeys; ++r) {
+            const int i = b0 + r * kBlk + t;
+            k[r] = i < n ? keys[i] : 0ull;
+        }
+#pragma unroll
+        for (int r = 0; r < kSortHistKeys; ++r) {
+            if (b0 + r * kBlk + t < n) {
+#pragma unroll
+                for (int p = 0; p < kSortPasses; ++p)
+                    atomicAdd(&hc[p * kRow + (int)((k[r] >> (8 * p)) & 255ull)], 1);
+            }
+        }
     }
     __syncthreads();
 #pragma unroll
-    for (int p = 0; p < kSortPasses; ++p)
-        if (h[p][t]) atomicAdd(&ss->hist[p][t], h[p][t]);
+    for (int p = 0; p < kSortPasses; ++p) {
+        int c = 0;
+#pragma unroll
+        for (int q = 0; q < kSortHistCopies; ++q) c += h[(q * kSortPasses + p) * kRow + t];
+        if (c) atomicAdd(&ss->hist[p][t], c);
+    }
 }
 
 // Exclusive scan of every column of a column-major count table
@@ -456,27 +482,45 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
             loff += sred[0][k];
             dstart += sred[1][k];
         }
-        // Decoupled look-back for digit t.
+        // Decoupled look-back for digit t, kLookWindow earlier tiles per
+        // memory round trip: their words are consumed nearest first until a
+        // tile with an inclusive prefix; an unpublished word stalls the walk,
+        // which resumes at that tile.
         int excl = 0;
         if (tile > 0) {
             int tt = tile - 1;
             int spins = 0;
-            for (;;) {
-                const unsigned v = __hip_atomic_load(look + (size_t)tt * kSortBins + t,
-                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned f = v >> 30;
-                if (f == 0u) {
+            bool done = false;
+            while (!done) {
+                unsigned v[kLookWindow];
+#pragma unroll
+                for (int j = 0; j < kLookWindow; ++j)
+                    v[j] = __hip_atomic_load(look + (size_t)max(tt - j, 0) * kSortBins + t,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int used = 0;
+                bool stall = false;
+#pragma unroll
+                for (int j = 0; j < kLookWindow; ++j) {
+                    if (!done && !stall) {
+                        const unsigned f = v[j] >> 30;
+                        if (f == 0u) {
+                            stall = true;
+                        } else {
+                            excl += (int)(v[j] & 0x3fffffffu);
+                            ++used;
+                            done = f == 2u || tt - j == 0;
+                        }
+                    }
+                }
+                tt -= used;
+                if (stall && !done) {
                     if (++spins > kLookbackSpins) {
                         atomicOr(&ctr->overflow, 8);
                         sbad = 1;
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
-                    continue;
                 }
-                excl += (int)(v & 0x3fffffffu);
-                if (f == 2u || tt == 0) break;
-                --tt;
             }
             __hip_atomic_store(my, (2u << 30) | (unsigned)(excl + c), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -775,19 +819,20 @@ __global__ __launch_bounds__(kBlk) void k_emit(
 
 
 // ------------------------------------------------------------------ up-sweep
-// Moments of node `id` (FP64): a leaf from its particles, an internal node
-// from its (already finished) children in child order.  Writes the FP64
-// record and the FP32 walk copies.
-__device__ void node_moments(int id, int n, int total, const RootInfo* root,
+// Moments of node `id` (FP64): a leaf (kLeaf) from its particles, an
+// internal node from its (already finished) children in child order.  Writes
+// the FP64 record and the FP32 walk copies.
+template <bool kLeaf>
+__device__ __forceinline__ void node_moments(int id, int n, int total, const RootInfo* root,
                              const int* __restrict__ pstart, const int* __restrict__ pend,
                              const int* child, const int* cend,
                              const signed char* __restrict__ level,
                              const float4* __restrict__ spos, double* mom, float4* mac,
                              int2* link, float4* q0, float4* q1, double inv_theta2, int quad,
                              Counters* ctr) {
-    const int c0 = child[id];
+    const int c0 = child[id];  // < 0 for a leaf (kLeaf), >= 0 otherwise: the caller checks
     // Structural checks: a malformed tree is reported, never dereferenced.
-    if (c0 < 0) {
+    if constexpr (kLeaf) {
         const int a = pstart[id], b = pend[id];
         if (a < 0 || b > n || a >= b) {
             atomicOr(&ctr->overflow, 4);
@@ -802,21 +847,31 @@ __device__ void node_moments(int id, int n, int total, const RootInfo* root,
             return;
         }
     }
+    // Every load of a batch is issued before the first is used: indices past
+    // the end repeat the last particle (child) and are not counted.  The sums
+    // run in particle (child) order.
     double M = 0, cx = 0, cy = 0, cz = 0;
     double Q[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz
-    if (c0 < 0) {
+    if constexpr (kLeaf) {
         const int a = pstart[id], b = pend[id];
         double sx = 0, sy = 0, sz = 0, ux = 0, uy = 0, uz = 0;
-        for (int q = a; q < b; ++q) {
-            const float4 p = spos[q];
-            const double m = p.w;
-            M += m;
-            sx += m * p.x;
-            sy += m * p.y;
-            sz += m * p.z;
-            ux += p.x;
-            uy += p.y;
-            uz += p.z;
+        for (int q0 = a; q0 < b; q0 += kMomBatch) {
+            float4 pb[kMomBatch];
+#pragma unroll
+            for (int k = 0; k < kMomBatch; ++k) pb[k] = spos[min(q0 + k, b - 1)];
+#pragma unroll
+            for (int k = 0; k < kMomBatch; ++k) {
+                if (q0 + k < b) {
+                    const double m = pb[k].w;
+                    M += m;
+                    sx += m * pb[k].x;
+                    sy += m * pb[k].y;
+                    sz += m * pb[k].z;
+                    ux += pb[k].x;
+                    uy += pb[k].y;
+                    uz += pb[k].z;
+                }
+            }
         }
         if (M > 0) {
             cx = sx / M;
@@ -829,31 +884,50 @@ __device__ void node_moments(int id, int n, int total, const RootInfo* root,
             cz = uz * inv;
         }
         if (quad) {
-            for (int q = a; q < b; ++q) {
-                const float4 p = spos[q];
-                const double m = p.w;
-                const double dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                Q[0] += m * (3.0 * dx * dx - r2);
-                Q[1] += m * (3.0 * dy * dy - r2);
-                Q[2] += m * (3.0 * dz * dz - r2);
-                Q[3] += m * (3.0 * dx * dy);
-                Q[4] += m * (3.0 * dx * dz);
-                Q[5] += m * (3.0 * dy * dz);
+            for (int q0 = a; q0 < b; q0 += kMomBatch) {
+                float4 pb[kMomBatch];
+#pragma unroll
+                for (int k = 0; k < kMomBatch; ++k) pb[k] = spos[min(q0 + k, b - 1)];
+#pragma unroll
+                for (int k = 0; k < kMomBatch; ++k) {
+                    if (q0 + k < b) {
+                        const double m = pb[k].w;
+                        const double dx = pb[k].x - cx, dy = pb[k].y - cy, dz = pb[k].z - cz;
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        Q[0] += m * (3.0 * dx * dx - r2);
+                        Q[1] += m * (3.0 * dy * dy - r2);
+                        Q[2] += m * (3.0 * dz * dz - r2);
+                        Q[3] += m * (3.0 * dx * dy);
+                        Q[4] += m * (3.0 * dx * dz);
+                        Q[5] += m * (3.0 * dy * dz);
+                    }
+                }
             }
         }
     } else {
+        // At most 8 children (checked above): one batch.  A record is
+        // (M, cx | cy, cz | Qxx, Qyy | Qzz, Qxy | Qxz, Qyz), 16-byte aligned.
         const int c1 = cend[id];
+        double2 rc[8][5];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double2* r = reinterpret_cast<const double2*>(mom + (size_t)min(c0 + k, c1 - 1) * 10);
+#pragma unroll
+            for (int j = 0; j < 5; ++j) rc[k][j] = r[j];
+        }
         double sx = 0, sy = 0, sz = 0, ux = 0, uy = 0, uz = 0;
-        for (int c = c0; c < c1; ++c) {
-            const double* mc = mom + (size_t)c * 10;
-            M += mc[0];
-            sx += mc[0] * mc[1];
-            sy += mc[0] * mc[2];
-            sz += mc[0] * mc[3];
-            ux += mc[1];
-            uy += mc[2];
-            uz += mc[3];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (c0 + k < c1) {
+                const double m = rc[k][0].x;
+                M += m;
+                sx += m * rc[k][0].y;
+                sy += m * rc[k][1].x;
+                sz += m * rc[k][1].y;
+                ux += rc[k][0].y;
+                uy += rc[k][1].x;
+                uz += rc[k][1].y;
+            }
         }
         if (M > 0) {
             cx = sx / M;
@@ -866,31 +940,32 @@ __device__ void node_moments(int id, int n, int total, const RootInfo* root,
             cz = uz * inv;
         }
         if (quad) {
-            for (int c = c0; c < c1; ++c) {
-                const double* mc = mom + (size_t)c * 10;
-                const double m = mc[0];
-                const double dx = mc[1] - cx, dy = mc[2] - cy, dz = mc[3] - cz;
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                Q[0] += mc[4] + m * (3.0 * dx * dx - r2);
-                Q[1] += mc[5] + m * (3.0 * dy * dy - r2);
-                Q[2] += mc[6] + m * (3.0 * dz * dz - r2);
-                Q[3] += mc[7] + m * (3.0 * dx * dy);
-                Q[4] += mc[8] + m * (3.0 * dx * dz);
-                Q[5] += mc[9] + m * (3.0 * dy * dz);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (c0 + k < c1) {
+                    const double m = rc[k][0].x;
+                    const double dx = rc[k][0].y - cx, dy = rc[k][1].x - cy, dz = rc[k][1].y - cz;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    Q[0] += rc[k][2].x + m * (3.0 * dx * dx - r2);
+                    Q[1] += rc[k][2].y + m * (3.0 * dy * dy - r2);
+                    Q[2] += rc[k][3].x + m * (3.0 * dz * dz - r2);
+                    Q[3] += rc[k][3].y + m * (3.0 * dx * dy);
+                    Q[4] += rc[k][4].x + m * (3.0 * dx * dz);
+                    Q[5] += rc[k][4].y + m * (3.0 * dy * dz);
+                }
             }
         }
     }
-    double* mo = mom + (size_t)id * 10;
-    mo[0] = M;
-    mo[1] = cx;
-    mo[2] = cy;
-    mo[3] = cz;
-#pragma unroll
-    for (int a = 0; a < 6; ++a) mo[4 + a] = Q[a];
+    double2* mo = reinterpret_cast<double2*>(mom + (size_t)id * 10);
+    mo[0] = make_double2(M, cx);
+    mo[1] = make_double2(cy, cz);
+    mo[2] = make_double2(Q[0], Q[1]);
+    mo[3] = make_double2(Q[2], Q[3]);
+    mo[4] = make_double2(Q[4], Q[5]);
     const double size = ldexp(root->size, -(int)level[id]);
     const double r2open = size * size * inv_theta2;
     mac[id] = make_float4((float)cx, (float)cy, (float)cz, (float)r2open);
-    link[id] = c0 < 0 ? make_int2(~pstart[id], pend[id]) : make_int2(c0, cend[id]);
+    link[id] = kLeaf ? make_int2(~pstart[id], pend[id]) : make_int2(c0, cend[id]);
     q0[id] = make_float4((float)M, (float)Q[0], (float)Q[1], (float)Q[2]);
     q1[id] = make_float4((float)Q[3], (float)Q[4], (float)Q[5], 0.f);
 }
@@ -906,7 +981,7 @@ __global__ __launch_bounds__(kBlk) void k_leaf_moments(
     const int total = li->total;
     for (int id = blockIdx.x * kBlk + threadIdx.x; id < total; id += gridDim.x * kBlk)
         if (child[id] < 0)
-            node_moments(id, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
+            node_moments<true>(id, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
                          link, q0, q1, inv_theta2, quad, ctr);
 }
 
@@ -927,7 +1002,7 @@ __global__ __launch_bounds__(kBlk) void k_upsweep(
     const int total = li->total;
     for (int k = blockIdx.x * kBlk + threadIdx.x; k < cnt; k += gridDim.x * kBlk)
         if (child[st + k] >= 0)
-            node_moments(st + k, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
+            node_moments<false>(st + k, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
                          link, q0, q1, inv_theta2, quad, ctr);
 }
 
@@ -1376,8 +1451,8 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     SortState* ss = w->sortst.as<SortState>();
     unsigned* look = w->hist.as<unsigned>();
     const int nlook = nsort * kSortBins * kSortPasses;
-    hipLaunchKernelGGL(k_sort_hist, dim3(std::min(nsort * kSortRows, kSortHistBlocks)), dim3(kBlk), 0,
-                       s, n, k0, ss, look, nlook);
+    hipLaunchKernelGGL(k_sort_hist, dim3(std::min(grid_for(n, kBlk * kSortHistKeys), kSortHistBlocks)),
+                       dim3(kBlk), 0, s, n, k0, ss, look, nlook);
     for (int p = 0; p < kSortPasses; ++p) {
         hipLaunchKernelGGL(k_onesweep, dim3(nsort), dim3(kOsThreads), 0, s, n, k0, v0, k1, v1, p, ss,
                            look + (size_t)p * nsort * kSortBins, ctr);
@@ -1418,9 +1493,10 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->w_q0.as<float4>(), w->w_q1.as<float4>(), inv_theta2,
                        multipole == NB_QUADRUPOLE ? 1 : 0, n, ctr);
     for (int l = kLevels - 1; l >= 0; --l) {
-        // One thread per node: a level has at most min(8^l, n) nodes.
+        // One thread per node: a level has at most min(8^l, n) nodes.  The
+        // grid is capped (the kernel strides): deep levels are mostly empty.
         const int64_t lvl_max = l < 8 ? std::min<int64_t>((int64_t)1 << (3 * l), n) : n;
-        const int ugrid = (int)std::max<int64_t>(1, (lvl_max + kBlk - 1) / kBlk);
+        const int ugrid = (int)std::min<int64_t>(kUpsweepBlocks, std::max<int64_t>(1, (lvl_max + kBlk - 1) / kBlk));
         hipLaunchKernelGGL(k_upsweep, dim3(ugrid), dim3(kBlk), 0, s, l, li, root,
                            w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
                            w->n_cend.as<int>(), w->n_level.as<signed char>(),
