@@ -239,6 +239,44 @@ def test_two_distant_clusters(nb, O):
     assert np.percentile(e, 99) < 1e-3
 
 
+def test_massless_tracers(nb, O):
+    """Zero-mass particles (tracers) exert no force and feel the others' field.
+    A node whose particles are all massless gets zero moments and a centre
+    inside its particles' bounding box (a leaf: their centroid; an internal
+    node: its children's), so every acceleration stays finite.  (The reference's incremental
+    centre of mass, CPU/src/force_calculator_barnes_hut_optimized.cpp:134-135,
+    divides 0/0 when a node's first particle is massless.)"""
+    n = 20000
+    pl = nb.ic_plummer(n, 12)
+    rng = np.random.default_rng(12)
+    m, x, y, z = pl["m"].copy(), pl["x"].copy(), pl["y"].copy(), pl["z"].copy()
+    m[rng.random(n) < 0.25] = 0.0                       # scattered tracers
+    x[:300], y[:300], z[:300] = 3.0 + 0.01 * rng.normal(size=(3, 300))  # a massless clump
+    m[:300] = 0.0
+    d = dict(m=m, x=x, y=y, z=z)
+    f, p = run_bh(nb, d, eps=0.005, cap=8, G=1.0)
+    a = p.acc()
+    assert all(np.all(np.isfinite(v)) for v in a)
+    e = rel_err(a, f64_direct(nb, d, 0.005, 1.0))
+    assert np.percentile(e, 99) <= 1e-3 and np.median(e) <= 3e-4, (np.median(e), np.percentile(e, 99))
+    # Topology does not depend on mass; massless nodes lie among their particles.
+    top = f.topology(n)
+    dep, path, _, _ = O.bhopt_leaves(m, x, y, z, 8)
+    assert np.array_equal(top["depth"], dep) and np.array_equal(top["path"], path)
+    nodes = f.nodes()
+    P = np.stack([x, y, z], 1)[top["perm"]].astype(np.float32).astype(np.longdouble)
+    size0 = f.stats()["root_size"]
+    empty = np.flatnonzero(nodes["mass"] == 0.0)
+    assert len(empty) > 10
+    for k in empty:
+        a0, b0 = nodes["first"][k], nodes["last"][k]
+        tol = 1e-12 * size0 * 2.0 ** -nodes["level"][k]
+        lo = np.array(P[a0:b0].min(0), dtype=float) - tol
+        hi = np.array(P[a0:b0].max(0), dtype=float) + tol
+        assert np.all(lo <= nodes["com"][k]) and np.all(nodes["com"][k] <= hi)
+        assert np.all(nodes["quad"][k] == 0.0)
+
+
 def test_leapfrog_with_bh_conserves_energy(nb):
     d = nb.ic_plummer(32768, 12)
     f = nb.BarnesHutForceCalculator(0.01, 0.5, 8, "quadrupole", G=1.0)
