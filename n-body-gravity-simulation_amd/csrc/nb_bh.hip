@@ -21,9 +21,10 @@
 //             key pairs (j, j+cap) around it.  Nodes are numbered in level
 //             order (key order inside a level); children of a node are
 //             contiguous.  Empty octants are not materialised.
-//   up-sweep  k_upsweep per level (deepest first): mass, centre of mass and the
-//             traceless quadrupole in FP64 for every level; FP32 copies for
-//             the walk.
+//   up-sweep  k_leaf_moments (every leaf, from its particles), then k_upsweep
+//             per level, deepest first (internal nodes, from their children):
+//             mass, centre of mass and the traceless quadrupole in FP64 for
+//             every level; FP32 copies for the walk.
 //   walk      k_walk: persistent one-wave blocks take walk groups (the tree
 //             cells of at most 64 particles) from an atomic counter, in
 //             size classes largest first, key order inside a class (the order
