@@ -272,7 +272,8 @@ def bench_serve(args):
     [10, 1000], eps = 0.005, dt = 1e-4; direct method below N = 32,768,
     Barnes-Hut (theta = 0.5) above; HBM budget 200 GB.  Requests are submitted
     at their (replayed) arrival times while earlier ones run; reports p50/p99
-    completion latency (arrival -> result on the host) and aggregate
+    completion latency (arrival -> result on the host), split into queueing
+    (arrival -> admission) and service (admission -> result), and aggregate
     particle-steps/s per rate."""
     import nbody_amd as nb
 
@@ -290,6 +291,7 @@ def bench_serve(args):
         t0 = time.perf_counter()
         sub = 0
         done = {}
+        admitted = {}
         rid_of = {}
         while len(done) < per_rate:
             now = time.perf_counter() - t0
@@ -303,17 +305,27 @@ def bench_serve(args):
                 continue
             srv.tick()
             t = time.perf_counter() - t0
+            for rid in srv.poll_admitted():
+                admitted[rid_of[rid]] = t
             for rid in srv.poll():
                 k = rid_of[rid]
                 srv.result(rid)
                 done[k] = t - arrivals[k]
         wall = time.perf_counter() - t0
         lat = np.array([done[k] for k in range(per_rate)])
+        queue = np.array([admitted[k] - arrivals[k] for k in range(per_rate)])
+        service = lat - queue
         work = float(np.sum(ns * steps))
         rows.append({"rate_per_s": rate, "requests": per_rate, "p50_s": float(np.percentile(lat, 50)),
-                     "p99_s": float(np.percentile(lat, 99)), "particle_steps_per_s": work / wall,
+                     "p99_s": float(np.percentile(lat, 99)),
+                     "queue_p50_s": float(np.percentile(queue, 50)),
+                     "queue_p99_s": float(np.percentile(queue, 99)),
+                     "service_p50_s": float(np.percentile(service, 50)),
+                     "service_p99_s": float(np.percentile(service, 99)),
+                     "particle_steps_per_s": work / wall,
                      "wall_s": wall, "bh_requests": int(np.sum(ns >= 32768))})
-        log(f"[serve] rate {rate}: p50 {rows[-1]['p50_s']:.2f} s, p99 {rows[-1]['p99_s']:.2f} s, "
+        log(f"[serve] rate {rate}: p50 {rows[-1]['p50_s']:.2f} s, p99 {rows[-1]['p99_s']:.2f} s "
+            f"(queueing p50 {rows[-1]['queue_p50_s']:.2f} s, service p50 {rows[-1]['service_p50_s']:.2f} s), "
             f"{rows[-1]['particle_steps_per_s']:.3g} particle-steps/s")
     best = max(r["particle_steps_per_s"] for r in rows)
     return {

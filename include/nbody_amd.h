@@ -324,6 +324,11 @@ int nb_server_submit(nb_server* s, const nb_request* r, int64_t* id);
 int nb_server_tick(nb_server* s, int* n_completed);
 /* Ids of requests finished since the last poll. */
 int nb_server_poll(nb_server* s, int64_t* ids, int capacity, int* count);
+/* Ids of requests admitted (moved from pending to active, device state
+ * allocated) since the last call, in admission order, so that a caller can
+ * split a request's latency into queueing and service time.  The server
+ * keeps one id per admitted request until it is polled here. */
+int nb_server_poll_admitted(nb_server* s, int64_t* ids, int capacity, int* count);
 /* Final positions/velocities of a finished request (then forgotten). */
 int nb_server_result(nb_server* s, int64_t id, double* x, double* y, double* z, double* vx,
                      double* vy, double* vz);

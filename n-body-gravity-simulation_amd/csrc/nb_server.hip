@@ -191,6 +191,7 @@ struct nb_server {
     std::vector<std::unique_ptr<Request>> active;
     std::map<int64_t, std::unique_ptr<Request>> done;
     std::vector<int64_t> completed_since_poll;
+    std::vector<int64_t> admitted_since_poll;
     // Two launch slots (priming batch, stepping batch) per tick, each with its
     // own device tables and the host staging they are copied from (kept alive
     // until the tick's synchronisation).
@@ -237,6 +238,7 @@ int admit(nb_server* s) {
             NB_TRY(r->pack.reserve((size_t)r->desc.n * sizeof(float4)));
         }
         s->bytes_used += (int64_t)r->bytes;
+        s->admitted_since_poll.push_back(r->id);
         s->active.push_back(std::move(s->pending.front()));
         s->pending.pop_front();
     }
@@ -449,6 +451,15 @@ int nb_server_poll(nb_server* s, int64_t* ids, int capacity, int* count) {
     const int n = (int)std::min<size_t>((size_t)std::max(0, capacity), s->completed_since_poll.size());
     for (int k = 0; k < n; ++k) ids[k] = s->completed_since_poll[k];
     s->completed_since_poll.erase(s->completed_since_poll.begin(), s->completed_since_poll.begin() + n);
+    *count = n;
+    return NB_OK;
+}
+
+int nb_server_poll_admitted(nb_server* s, int64_t* ids, int capacity, int* count) {
+    if (!s || !count || (!ids && capacity > 0)) return NB_ERR_INVALID;
+    const int n = (int)std::min<size_t>((size_t)std::max(0, capacity), s->admitted_since_poll.size());
+    for (int k = 0; k < n; ++k) ids[k] = s->admitted_since_poll[k];
+    s->admitted_since_poll.erase(s->admitted_since_poll.begin(), s->admitted_since_poll.begin() + n);
     *count = n;
     return NB_OK;
 }

@@ -137,6 +137,7 @@ def _load():
         "nb_server_submit": ([_vp, C.POINTER(_Request), C.POINTER(C.c_int64)], C.c_int),
         "nb_server_tick": ([_vp, C.POINTER(C.c_int)], C.c_int),
         "nb_server_poll": ([_vp, _vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
+        "nb_server_poll_admitted": ([_vp, _vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "nb_server_result": ([_vp, C.c_int64] + [_Dn] * 6, C.c_int),
         "nb_server_get_status": ([_vp, C.POINTER(_ServerStatus)], C.c_int),
     }
@@ -573,6 +574,13 @@ class Server:
         ids = np.zeros(capacity, np.int64)
         c = C.c_int(0)
         _check(lib.nb_server_poll(self._h, ids.ctypes.data, capacity, C.byref(c)))
+        return ids[:c.value].tolist()
+
+    def poll_admitted(self, capacity=4096):
+        """Ids admitted (state resident on the device) since the last call."""
+        ids = np.zeros(capacity, np.int64)
+        c = C.c_int(0)
+        _check(lib.nb_server_poll_admitted(self._h, ids.ctypes.data, capacity, C.byref(c)))
         return ids[:c.value].tolist()
 
     def result(self, rid):

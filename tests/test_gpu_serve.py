@@ -82,7 +82,7 @@ def test_direct_request_vs_oracle(nb, O):
 def test_budget_admission(nb):
     sy = systems(nb)
     items = [(sy[0], 3), (sy[1], 3), (sy[3], 3)]
-    # Room for one direct request at a time: the others wait, then run.
+    # Room for the first two direct requests (96 kB + 288 kB of 432 kB): the third waits, then runs.
     budget = 3000 * 96 * 1.5
     res, srv = run_server(nb, items, budget=budget)
     alone = [run_server(nb, [it])[0][0] for it in items]
@@ -94,6 +94,33 @@ def test_budget_admission(nb):
     with pytest.raises(nb.NBError) as e:
         too_big.submit(sy[2], 1)
     assert e.value.code == 5  # NB_ERR_BUDGET
+
+
+def test_admission_order_and_queueing(nb):
+    """Every request is reported admitted exactly once, in submission order
+    (admission is first come, first served), and before it completes.  The
+    budget holds one of these direct requests at a time (96 B per particle:
+    96 kB, 288 kB and 67 kB against 300 kB), so each waits for the one before
+    it to finish."""
+    sy = systems(nb)
+    srv = nb.Server(hbm_budget_bytes=300e3, direct_threshold=32768, bh_lanes=1)
+    ids = [srv.submit(sy[0], 3), srv.submit(sy[1], 3), srv.submit(sy[3], 2)]
+    admitted, completed, tick_admitted, tick_done = [], [], {}, {}
+    for tick in range(1, 100):
+        srv.tick()
+        for rid in srv.poll_admitted():
+            admitted.append(rid)
+            tick_admitted[rid] = tick
+        for rid in srv.poll():
+            assert rid in tick_admitted
+            completed.append(rid)
+            tick_done[rid] = tick
+            srv.result(rid)
+        if len(completed) == len(ids):
+            break
+    assert admitted == ids and sorted(completed) == sorted(ids)
+    assert tick_admitted[ids[1]] >= tick_done[ids[0]] and tick_admitted[ids[2]] >= tick_done[ids[1]]
+    assert srv.poll_admitted() == []
 
 
 def test_bh_lane_workspace_accounting(nb):
