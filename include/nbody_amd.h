@@ -291,9 +291,11 @@ int nb_ensemble_tuning(const nb_ensemble* e, int32_t* shape_codes, int n_buckets
 /* ---------------------------------------------------------------- serving */
 /* Continuous-batching server: requests (one system each) are admitted against
  * an HBM budget, keep their state resident on the device, and every tick
- * advances every active request by one leapfrog step -- direct-method requests
- * (n < direct_threshold) in one batched launch, Barnes-Hut requests on a pool
- * of bh_lanes calculators with their own streams, overlapping on the device.
+ * advances every active request -- direct-method requests (n <
+ * direct_threshold) together in batched launches, K leapfrog steps per tick
+ * (K = 4 unless set by nb_server_set_direct_steps, and no more than any of
+ * them has left), Barnes-Hut requests one step each on a pool of bh_lanes
+ * calculators with their own streams, overlapping on the device.
  * Finished requests are downloaded and released; pending ones are admitted as
  * memory frees up.  Budget accounting: every active request is charged its
  * device state (80 B/particle; direct requests 16 B/particle more for the FP32
@@ -333,6 +335,10 @@ int nb_server_poll_admitted(nb_server* s, int64_t* ids, int capacity, int* count
 int nb_server_result(nb_server* s, int64_t id, double* x, double* y, double* z, double* vx,
                      double* vy, double* vz);
 int nb_server_get_status(const nb_server* s, nb_server_status* out);
+/* Direct-request steps per tick, 1..64 (default 4).  More steps per tick
+ * finish small requests sooner while tree steps of large ones share the
+ * device; results do not depend on it (each step is the same arithmetic). */
+int nb_server_set_direct_steps(nb_server* s, int steps_per_tick);
 
 #ifdef __cplusplus
 }

@@ -2,10 +2,13 @@
 //
 // Requests (one N-body system each, with a step count) are admitted against
 // an HBM budget and keep their state resident on the device until they
-// finish.  Every tick advances each active request by one leapfrog step:
-//   * direct-method requests (N below a threshold) are stepped together: one
-//     batched kick-drift launch, one persistent force launch over all of them
-//     (work items (request, target tile, shape) pulled from an atomic
+// finish.  Every tick advances each Barnes-Hut request by one leapfrog step
+// and each direct request by K steps, K = min(the direct steps per tick, 4
+// unless set, the fewest steps any direct request has left), so that cheap
+// requests are not paced by the tree steps in the same tick:
+//   * direct-method requests (N below a threshold) are stepped together: per
+//     step one batched kick-drift launch, one persistent force launch over all
+//     of them (work items (request, target tile, shape) pulled from an atomic
 //     counter), one closing-kick launch -- all on the server stream;
 //   * Barnes-Hut requests are spread over a fixed pool of BH lanes, each a
 //     force calculator with its own stream and workspace, so tree builds and
@@ -29,6 +32,8 @@
 namespace nb {
 
 constexpr int kSrvBlock = 256;
+constexpr int kDirectStepsPerTick = 4;  // default direct-request steps per tick (fewer near their end)
+constexpr int kDirectStepsPerTickMax = 64;
 
 struct SrvSys {
     const float4* pack;
@@ -192,6 +197,7 @@ struct nb_server {
     std::map<int64_t, std::unique_ptr<Request>> done;
     std::vector<int64_t> completed_since_poll;
     std::vector<int64_t> admitted_since_poll;
+    int direct_steps = kDirectStepsPerTick;  // direct-request steps per tick
     // Two launch slots (priming batch, stepping batch) per tick, each with its
     // own device tables and the host staging they are copied from (kept alive
     // until the tick's synchronisation).
@@ -247,8 +253,9 @@ int admit(nb_server* s) {
 
 int shape_for(int n) { return n >= 2048 ? 2 : n >= 512 ? 1 : 0; }
 
-// One batched leapfrog step (or only the priming force) of the direct requests.
-int direct_step(nb_server* s, const std::vector<Request*>& reqs, bool prime_only) {
+// `nsteps` batched leapfrog steps (or only the priming force) of the direct
+// requests; the launch tables are built once and serve every step.
+int direct_step(nb_server* s, const std::vector<Request*>& reqs, bool prime_only, int nsteps = 1) {
     if (reqs.empty()) return NB_OK;
     nb_server::Slot& sl = s->slot[prime_only ? 0 : 1];
     std::vector<SrvSys>& sys = sl.sys;
@@ -298,12 +305,14 @@ int direct_step(nb_server* s, const std::vector<Request*>& reqs, bool prime_only
                            sl.d_kick.as<SrvKick>(), 2);
         return force();
     }
-    hipLaunchKernelGGL(k_srv_kick, kg, dim3(kSrvBlock), 0, s->stream, sl.d_kitems.as<SrvKickItem>(),
-                       sl.d_kick.as<SrvKick>(), 0);
-    NB_TRY(force());
-    hipLaunchKernelGGL(k_srv_kick, kg, dim3(kSrvBlock), 0, s->stream, sl.d_kitems.as<SrvKickItem>(),
-                       sl.d_kick.as<SrvKick>(), 1);
-    NB_LAUNCH_CHECK();
+    for (int k = 0; k < nsteps; ++k) {
+        hipLaunchKernelGGL(k_srv_kick, kg, dim3(kSrvBlock), 0, s->stream, sl.d_kitems.as<SrvKickItem>(),
+                           sl.d_kick.as<SrvKick>(), 0);
+        NB_TRY(force());
+        hipLaunchKernelGGL(k_srv_kick, kg, dim3(kSrvBlock), 0, s->stream, sl.d_kitems.as<SrvKickItem>(),
+                           sl.d_kick.as<SrvKick>(), 1);
+        NB_LAUNCH_CHECK();
+    }
     return NB_OK;
 }
 
@@ -414,17 +423,20 @@ int nb_server_tick(nb_server* s, int* n_completed) {
             if (!r->have_acc) fresh.push_back(r);
         }
     }
+    int kdir = s->direct_steps;
+    for (Request* r : direct) kdir = std::min(kdir, r->desc.steps - r->done);
     NB_TRY(direct_step(s, fresh, true));
     for (Request* r : fresh) r->have_acc = true;
-    NB_TRY(direct_step(s, direct, false));
+    NB_TRY(direct_step(s, direct, false, kdir));
     NB_HIP(hipDeviceSynchronize());
     int nc = 0;
     std::vector<std::unique_ptr<Request>> still;
     for (auto& up : s->active) {
         Request* r = up.get();
         if (r->done < r->desc.steps) {
-            r->done++;
-            s->particle_steps += r->desc.n;
+            const int k = r->bh ? 1 : kdir;
+            r->done += k;
+            s->particle_steps += r->desc.n * k;
         }
         if (r->done >= r->desc.steps) {
             for (int k = 1; k < 7; ++k) r->host[k].resize(r->desc.n);
@@ -477,6 +489,16 @@ int nb_server_result(nb_server* s, int64_t id, double* x, double* y, double* z, 
     for (int k = 0; k < 6; ++k)
         if (dst[k]) std::memcpy(dst[k], r->host[k + 1].data(), sizeof(double) * r->desc.n);
     s->done.erase(it);
+    return NB_OK;
+}
+
+int nb_server_set_direct_steps(nb_server* s, int steps_per_tick) {
+    if (!s || steps_per_tick < 1 || steps_per_tick > kDirectStepsPerTickMax) {
+        set_error("nb_server_set_direct_steps: steps_per_tick=%d outside [1, %d]", steps_per_tick,
+                  kDirectStepsPerTickMax);
+        return NB_ERR_INVALID;
+    }
+    s->direct_steps = steps_per_tick;
     return NB_OK;
 }
 

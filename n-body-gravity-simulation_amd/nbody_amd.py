@@ -140,6 +140,7 @@ def _load():
         "nb_server_poll_admitted": ([_vp, _vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "nb_server_result": ([_vp, C.c_int64] + [_Dn] * 6, C.c_int),
         "nb_server_get_status": ([_vp, C.POINTER(_ServerStatus)], C.c_int),
+        "nb_server_set_direct_steps": ([_vp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -549,12 +550,14 @@ class Server:
     """Continuous-batching server (see nb_server_* in include/nbody_amd.h)."""
 
     def __init__(self, hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=8,
-                 leaf_capacity=8):
+                 leaf_capacity=8, direct_steps_per_tick=None):
         h = C.c_void_p()
         _check(lib.nb_server_create(C.byref(h), float(hbm_budget_bytes), int(direct_threshold),
                                     int(bh_lanes), int(leaf_capacity)))
         self._h = h
         self._n = {}
+        if direct_steps_per_tick is not None:
+            _check(lib.nb_server_set_direct_steps(h, int(direct_steps_per_tick)))
 
     def submit(self, system, steps, G=1.0, softening=0.005, dt=1e-4, theta=0.5):
         arr = [_f64(system[k]) for k in ("m", "x", "y", "z", "vx", "vy", "vz")]

@@ -20,8 +20,9 @@ def systems(nb):
     return [nb.ic_plummer(n, seed) for n, _, seed in REQS]
 
 
-def run_server(nb, items, stagger=0, budget=200e9, lanes=2):
-    srv = nb.Server(hbm_budget_bytes=budget, direct_threshold=32768, bh_lanes=lanes)
+def run_server(nb, items, stagger=0, budget=200e9, lanes=2, kdir=None):
+    srv = nb.Server(hbm_budget_bytes=budget, direct_threshold=32768, bh_lanes=lanes,
+                    direct_steps_per_tick=kdir)
     ids = []
     pending = list(items)
     results = {}
@@ -121,6 +122,23 @@ def test_admission_order_and_queueing(nb):
     assert admitted == ids and sorted(completed) == sorted(ids)
     assert tick_admitted[ids[1]] >= tick_done[ids[0]] and tick_admitted[ids[2]] >= tick_done[ids[1]]
     assert srv.poll_admitted() == []
+
+
+def test_direct_steps_per_tick_does_not_change_results(nb):
+    """Direct requests take K steps per tick (bounded by the fewest steps any
+    of them has left); every K gives the same final states, bit for bit, and
+    K outside [1, 64] is rejected."""
+    sy = systems(nb)
+    items = [(sy[0], 5), (sy[1], 8), (sy[3], 6), (sy[4], 4)]
+    base, _ = run_server(nb, items, stagger=2, kdir=1)
+    for k in (3, 8):
+        res, srv = run_server(nb, items, stagger=2, kdir=k)
+        for a, b in zip(base, res):
+            assert all(np.array_equal(a[f], b[f]) for f in FIELDS)
+        assert srv.status()["particle_steps"] == sum(len(sd["m"]) * st for sd, st in items)
+    for bad in (0, 65):
+        with pytest.raises(nb.NBError):
+            nb.Server(direct_steps_per_tick=bad)
 
 
 def test_bh_lane_workspace_accounting(nb):
