@@ -368,7 +368,8 @@ int nb_server_destroy(nb_server* s) {
 
 int nb_server_submit(nb_server* s, const nb_request* req, int64_t* id) {
     if (!s || !req || !id || req->n <= 0 || req->steps < 0 || !req->m || !req->x || !req->y ||
-        !req->z || !req->vx || !req->vy || !req->vz || !(req->dt == req->dt)) {
+        !req->z || !req->vx || !req->vy || !req->vz || !(req->dt == req->dt) ||
+        !(req->theta >= 0.0) || !(req->softening >= 0.0)) {
         set_error("nb_server_submit: invalid request");
         return NB_ERR_INVALID;
     }
@@ -459,7 +460,7 @@ int nb_server_tick(nb_server* s, int* n_completed) {
 }
 
 int nb_server_poll(nb_server* s, int64_t* ids, int capacity, int* count) {
-    if (!s || !count) return NB_ERR_INVALID;
+    if (!s || !count || (!ids && capacity > 0)) return NB_ERR_INVALID;
     const int n = (int)std::min<size_t>((size_t)std::max(0, capacity), s->completed_since_poll.size());
     for (int k = 0; k < n; ++k) ids[k] = s->completed_since_poll[k];
     s->completed_since_poll.erase(s->completed_since_poll.begin(), s->completed_since_poll.begin() + n);

@@ -45,6 +45,10 @@ def test_invalid_arguments_are_rejected_before_device_use(nb):
     assert nb.lib.nb_integrator_create(C.byref(h), 9) == 1
     p = nb._Particles()
     assert nb.lib.nb_particles_alloc(C.byref(p), -5) == 1
+    cnt = C.c_int(0)
+    assert nb.lib.nb_server_poll(None, None, 0, C.byref(cnt)) == 1
+    assert nb.lib.nb_server_poll_admitted(None, None, 0, C.byref(cnt)) == 1
+    assert nb.lib.nb_server_set_direct_steps(None, 4) == 1
     with pytest.raises(nb.NBError):
         nb.ic_plummer(10, 1, a=-1.0)
 
