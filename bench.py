@@ -209,7 +209,7 @@ def bench_direct(args):
                      "traffic": traffic_bytes("k_direct_f32"),
                      "traffic_source": traffic_source("k_direct_f32"),
                      "launch_ms": round(kern_ms, 4), "flops_per_launch": FLOP_PP * n * (n - 1),
-                     "fma_probe": fma_probe(nb, achieved)},
+                     "fma_probe": fma_probe(nb, achieved, {"pp": 1})},
         "accuracy": {"median_rel_vs_f64": float(np.median(err)),
                      "p99_rel_vs_f64": float(np.percentile(err, 99)),
                      "energy_drift_rel": float(abs(E1 - E0) / abs(E0))},
@@ -340,14 +340,34 @@ def bench_serve(args):
     }
 
 
-def fma_probe(nb, achieved):
+# Instructions of the compiled interaction terms besides their one v_rsq_f32
+# (gfx950 ISA of the inner loops): the direct FP32 term with softening and the
+# walk's particle term are 3 sub + 3 mul + 6 FMA (20 FLOP); the walk's
+# quadrupole cell term is 3 add/sub + 9 mul + 18 FMA (60 FLOP).
+TERM_OTHER_INSTR = {"pp": 12, "pc": 30}
+TERM_FLOP = {"pp": 20, "pc": 60}
+
+
+def fma_probe(nb, achieved, terms=None):
     """The FP32 ceiling without packed FP32 (the library is built without it):
     v_fma_f32 issue rate with VGPR operands, measured in this run outside the
-    timed region, and the kernel's FLOP rate as a fraction of it."""
+    timed region, and the kernel's FLOP rate as a fraction of it.  With
+    `terms` ({"pp": count, "pc": count}), also the ceiling of that term mix:
+    the FMA slot rate (probe / 2 FLOP) times FLOP per slot, a term taking its
+    other instructions plus the measured rsqrt cost in slots."""
     r = nb.probe_fp32_fma()
-    return {"tflops": round(r["tflops"], 2), "frac": round(achieved / r["tflops"], 4),
-            "what": "16 independent v_fma_f32 chains per lane, VGPR operands, 8 x 256-lane "
-                    "blocks per CU (nb_probe_fp32_fma)"}
+    out = {"tflops": round(r["tflops"], 2), "frac": round(achieved / r["tflops"], 4),
+           "what": "16 independent v_fma_f32 chains per lane, VGPR operands, 8 x 256-lane "
+                   "blocks per CU (nb_probe_fp32_fma)"}
+    if terms:
+        c = nb.probe_fp32_rsq_cost()
+        flop = sum(TERM_FLOP[k] * v for k, v in terms.items())
+        slots = sum((TERM_OTHER_INSTR[k] + c) * v for k, v in terms.items())
+        ceiling = r["tflops"] / 2.0 * flop / slots
+        out["rsq_fma_slots"] = round(c, 2)
+        out["mix_ceiling_tflops"] = round(ceiling, 2)
+        out["mix_frac"] = round(achieved / ceiling, 4)
+    return out
 
 
 def copy_probe(nbytes=1 << 28):
@@ -454,7 +474,8 @@ def main():
                 "flops_per_launch": int(flops),
                 "interactions_per_particle": {"pp": round(st["pp_interactions"] / n, 1),
                                               "pc": round(st["pc_interactions"] / n, 1)},
-                "fma_probe": fma_probe(nb, achieved)}
+                "fma_probe": fma_probe(nb, achieved, {"pp": st["pp_interactions"],
+                                                      "pc": st["pc_interactions"]})}
     stage_out = stage_report(stages, n, st["n_nodes"])
 
     out = {

@@ -57,6 +57,12 @@ const char* nb_version(void);
  * the benchmark reports it beside their FLOP rates.  Synchronous; *ms (may be
  * NULL) receives the timed launch's duration. */
 int nb_probe_fp32_fma(double* tflops, double* ms);
+/* Diagnostic with no counterpart in the reference: the issue cost of one
+ * v_rsq_f32 in FMA slots on the current device, from the same chains with one
+ * rsqrt before every 12 FMAs (a softened pair term's mix) timed against FMAs
+ * alone.  A term of k other instructions and one rsqrt therefore issues at
+ * most at k + 1 instructions per (k + *fma_slots) FMA slots.  Synchronous. */
+int nb_probe_fp32_rsq_cost(double* fma_slots);
 
 /* ---------------------------------------------------------- particle state */
 /* Structure-of-arrays device state, FP64 throughout.  Replaces ParticleArrays

@@ -88,6 +88,7 @@ def _load():
         "nb_last_error": ([], C.c_char_p),
         "nb_version": ([], C.c_char_p),
         "nb_probe_fp32_fma": ([C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+        "nb_probe_fp32_rsq_cost": ([C.POINTER(C.c_double)], C.c_int),
         "nb_device_count": ([C.POINTER(C.c_int)], C.c_int),
         "nb_set_device": ([C.c_int], C.c_int),
         "nb_synchronize": ([], C.c_int),
@@ -177,6 +178,14 @@ def probe_fp32_fma() -> dict:
     tf, ms = C.c_double(0.0), C.c_double(0.0)
     _check(lib.nb_probe_fp32_fma(C.byref(tf), C.byref(ms)))
     return {"tflops": tf.value, "ms": ms.value}
+
+
+def probe_fp32_rsq_cost() -> float:
+    """Issue cost of one v_rsq_f32 in FMA slots, measured in a softened pair
+    term's mix (one rsqrt per 12 FMAs) against FMAs alone."""
+    c = C.c_double(0.0)
+    _check(lib.nb_probe_fp32_rsq_cost(C.byref(c)))
+    return c.value
 
 
 def synchronize():

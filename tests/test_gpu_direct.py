@@ -266,6 +266,14 @@ def test_orbit_example_reproduces_reference_orbit_csv(nb, orbit_rows, tmp_path):
 
 
 
+def test_rsq_cost_probe(nb):
+    """The rsqrt issue cost the benchmark's term-mix ceilings use: more than
+    one FMA slot (a transcendental) and less than the 16 of a fully serial
+    quarter-rate unit with no overlap at all."""
+    c = nb.probe_fp32_rsq_cost()
+    assert 1.0 < c < 16.0, c
+
+
 def test_fp32_fma_probe_bounds_the_fp32_kernel(nb):
     """The issue-rate probe the benchmark reports beside the FP32 kernels:
     positive, below the published packed-FP32 peak (157.3 TFLOP/s, which this
