@@ -136,7 +136,7 @@ def main():
                                                                    "sample": sample(a, acc_names)}
         a = O.ref_bhrec(d["m"], d["x"], d["y"], d["z"], 0.5, 0.0)
         gold[f"bhrec/{fx}/eps0.0/theta0.5"] = {"sha256": digest(*a), "sample": sample(a, acc_names)}
-        for cap in (1, 4, 8, 16):
+        for cap in (1, 4, 8, 16, 33, 64):
             dep, path, nint, nodes = O.ref_bhopt_leaves(d["m"], d["x"], d["y"], d["z"], cap)
             gold[f"topology/{fx}/cap{cap}"] = {"sha256": digest(dep, path), "internal": nint,
                                                "pool_nodes": nodes,

@@ -34,7 +34,7 @@ def f64_direct(nb, d, eps, G):
 
 
 @pytest.mark.parametrize("fx", ["p100", "p1000", "c1"])
-@pytest.mark.parametrize("cap", [1, 4, 8, 16])
+@pytest.mark.parametrize("cap", [1, 4, 8, 16, 33, 64])
 def test_keys_perm_topology_bitwise(nb, O, golden, fixtures, fx, cap):
     d = fixtures[fx]
     f, p = run_bh(nb, d, cap=cap)
@@ -110,6 +110,19 @@ def test_accuracy_plummer_65536(nb, mp, p99, med):
     e = rel_err(p.acc(), f64_direct(nb, pl, 0.005, 1.0))
     assert np.percentile(e, 99) <= p99 and np.median(e) <= med, (np.median(e), np.percentile(e, 99))
     assert f.stats()["frontier_overflow"] == 0
+
+
+@pytest.mark.parametrize("cap", [1, 33, 64])
+def test_accuracy_across_leaf_capacities(nb, cap):
+    """Leaf capacities at the ends of the accepted range [1, 64] (64 is also
+    the walk's group size, so a full leaf is a whole group): same force bar
+    as capacity 8 on the same Plummer input."""
+    pl = nb.ic_plummer(65536, 3)
+    f, p = run_bh(nb, pl, eps=0.005, theta=0.5, cap=cap, G=1.0)
+    e = rel_err(p.acc(), f64_direct(nb, pl, 0.005, 1.0))
+    assert np.percentile(e, 99) <= 1e-3 and np.median(e) <= 3e-4, (np.median(e), np.percentile(e, 99))
+    st = f.stats()
+    assert st["frontier_overflow"] == 0 and st["oversize_cells"] == 0
 
 
 def test_more_accurate_than_reference_bh(nb, O, fixtures):

@@ -59,7 +59,7 @@ def test_bhrec_matches_compiled_reference(O, golden, fixtures, fx):
 
 
 @pytest.mark.parametrize("fx", ["p100", "p1000", "c1"])
-@pytest.mark.parametrize("cap", [1, 4, 8, 16])
+@pytest.mark.parametrize("cap", [1, 4, 8, 16, 33, 64])
 def test_topology_matches_compiled_reference(O, golden, fixtures, fx, cap):
     d = fixtures[fx]
     dep, path, nint, pool = O.bhopt_leaves(d["m"], d["x"], d["y"], d["z"], cap)
