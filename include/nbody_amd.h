@@ -161,11 +161,16 @@ typedef struct nb_bh_stats {
     int64_t pc_interactions;   /* particle-cell interactions in the walk */
     int32_t max_frontier;      /* largest walk frontier seen */
     int32_t frontier_overflow; /* error flags, nonzero = result invalid: 1 walk frontier full,
-                                  2 node pool full, 4 malformed tree, 8 sort look-back wait limit */
+                                  2 node pool full, 4 malformed tree, 8 sort look-back wait limit,
+                                  16 non-finite position or mass.  With 2, 4, 8 or 16 the tree is
+                                  not walked and the accelerations are set to NaN. */
     double root_center[3];
     double root_size;
     int64_t n_groups;          /* walk groups (tree cells of <= 64 particles) */
 } nb_bh_stats;
+/* Fills *out and returns NB_OK, or (with *out filled) NB_ERR_INVALID if the
+ * last evaluation had a non-finite input (flag 16), NB_ERR_CAPACITY for the
+ * other flags, NB_ERR_STATE before any evaluation.  Synchronizes. */
 int nb_force_bh_stats(nb_force* f, nb_bh_stats* out);
 
 /* Topology of the last Barnes-Hut tree, per particle in ORIGINAL order: depth

@@ -118,7 +118,8 @@ struct SortState {
 };
 
 // Errors that make the sorted order (and so every later stage) invalid.
-constexpr int kTreeBad = 2 | 4 | 8;
+constexpr int kInputBad = 16;  // a non-finite position or mass
+constexpr int kTreeBad = 2 | 4 | 8 | kInputBad;
 
 }  // namespace nb
 
@@ -273,13 +274,19 @@ __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__
                                                const RootInfo* __restrict__ root,
                                                unsigned long long* __restrict__ keys,
                                                unsigned* __restrict__ vals,
-                                               float4* __restrict__ upos) {
+                                               float4* __restrict__ upos,
+                                               Counters* __restrict__ ctr) {
     const int i = blockIdx.x * kBlk + threadIdx.x;
     if (i >= n) return;
     double cx = root->center[0], cy = root->center[1], cz = root->center[2];
     double s = root->size;
-    const double px = x[i], py = y[i], pz = z[i];
-    upos[i] = make_float4((float)px, (float)py, (float)pz, (float)m[i]);
+    const double px = x[i], py = y[i], pz = z[i], pm = m[i];
+    upos[i] = make_float4((float)px, (float)py, (float)pz, (float)pm);
+    // A non-finite position or mass makes the evaluation invalid (error bit
+    // 16): the up-sweep and walk skip it.  One report per wave.
+    const bool bad = !(isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(pm));
+    const unsigned long long bb = __ballot(bad);
+    if (bb && lane_id() == __ffsll((long long)bb) - 1) atomicOr(&ctr->overflow, kInputBad);
     unsigned long long k = 0;
 #pragma unroll
     for (int l = 0; l < kMaxDigits; ++l) {
@@ -1026,8 +1033,17 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                                              const int* __restrict__ gorder,
                                              const LevelInfo* __restrict__ li,
                                              Counters* __restrict__ ctr) {
-    // A tree that failed the build/up-sweep checks is not walked.
-    if (ctr->overflow & kTreeBad) return;
+    // A tree that failed the input, sort, build or up-sweep checks is not
+    // walked; this evaluation's accelerations are NaN rather than stale.
+    if (ctr->overflow & kTreeBad) {
+        const double nan = __builtin_nan("");
+        for (int i = blockIdx.x * 64 + threadIdx.x; i < n; i += gridDim.x * 64) {
+            ax[i] = nan;
+            ay[i] = nan;
+            az[i] = nan;
+        }
+        return;
+    }
     __shared__ float4 pl[128];                 // staged source particles
     __shared__ float4 sc0[64], sc1[64], sc2[64];  // staged cells: centre of mass, (M Qxx Qyy Qzz), (Qxy Qxz Qyz)
     __shared__ int2 fl[2][64];                 // frontier entries 0..63 of this level / the next
@@ -1446,7 +1462,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        root, ctr, w->sortst.as<SortState>());
     if (ev && ev[1]) NB_HIP(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(k_keys, dim3(nblk), dim3(kBlk), 0, s, n, x, y, z, m, root, k0, v0,
-                       w->upos.as<float4>());
+                       w->upos.as<float4>(), ctr);
     NB_LAUNCH_CHECK();
     if (ev && ev[2]) NB_HIP(hipEventRecord(ev[2], s));
     SortState* ss = w->sortst.as<SortState>();
@@ -1569,10 +1585,10 @@ int bh_stats(BHWorkspace* w, hipStream_t s, nb_bh_stats* out) {
     out->root_size = r.size;
     out->n_groups = li.n_groups;
     if (c.overflow) {
-        set_error("Barnes-Hut: capacity exceeded or malformed tree (flags %d: 1=walk frontier, "
-                  "2=node pool, 4=tree consistency, 8=sort look-back wait)",
+        set_error("Barnes-Hut: invalid evaluation (flags %d: 1=walk frontier, 2=node pool, "
+                  "4=tree consistency, 8=sort look-back wait, 16=non-finite position or mass)",
                   c.overflow);
-        return NB_ERR_CAPACITY;
+        return (c.overflow & kInputBad) ? NB_ERR_INVALID : NB_ERR_CAPACITY;
     }
     return NB_OK;
 }

@@ -290,6 +290,33 @@ def test_massless_tracers(nb, O):
         assert np.all(nodes["quad"][k] == 0.0)
 
 
+@pytest.mark.parametrize("bad", ["x_inf", "z_nan", "m_nan"])
+def test_non_finite_input_is_reported(nb, bad):
+    """A non-finite position or mass sets error bit 16: the tree is not walked,
+    the accelerations of that evaluation are NaN (never stale), stats() raises,
+    and the next valid input evaluates exactly as on a fresh calculator."""
+    d = nb.ic_plummer(5000, 4)
+    good = {k: d[k].copy() for k in ("m", "x", "y", "z")}
+    field, val = {"x_inf": ("x", np.inf), "z_nan": ("z", np.nan), "m_nan": ("m", np.nan)}[bad]
+    d = dict(good)
+    d[field] = d[field].copy()
+    d[field][1234] = val
+    f = nb.BarnesHutForceCalculator(0.01, 0.5, 8, "quadrupole", G=1.0)
+    p = nb.Particles.from_arrays(d["m"], d["x"], d["y"], d["z"])
+    f.computeAccelerations(p)
+    assert all(np.all(np.isnan(v)) for v in p.acc())
+    st = f.stats(check=False)
+    assert st["frontier_overflow"] & 16 and st["status"] != 0
+    with pytest.raises(nb.NBError) as e:
+        f.stats()
+    assert e.value.code == 1  # NB_ERR_INVALID
+    q = nb.Particles.from_arrays(good["m"], good["x"], good["y"], good["z"])
+    f.computeAccelerations(q)
+    assert f.stats()["frontier_overflow"] == 0
+    _, fresh = run_bh(nb, good, eps=0.01, cap=8, G=1.0)
+    assert all(np.array_equal(a, b) for a, b in zip(q.acc(), fresh.acc()))
+
+
 def test_leapfrog_with_bh_conserves_energy(nb):
     d = nb.ic_plummer(32768, 12)
     f = nb.BarnesHutForceCalculator(0.01, 0.5, 8, "quadrupole", G=1.0)
