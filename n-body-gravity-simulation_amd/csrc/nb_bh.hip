@@ -1363,11 +1363,21 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
 }
 
 // ------------------------------------------------------------------ host
+// Node pool size: an upper bound on the tree's nodes (every particle a leaf
+// on a chain of up to 21 levels per cap + 1 particles).  NB_BH_NODE_CAP
+// (testing) can only lower it, to exercise the overflow report (error bit 2).
 static int64_t node_capacity(int64_t n, int cap) {
-    return n + (int64_t)kMaxDigits * ((n + cap) / (cap + 1)) + 1;
+    const int64_t bound = n + (int64_t)kMaxDigits * ((n + cap) / (cap + 1)) + 1;
+    if (const char* e = std::getenv("NB_BH_NODE_CAP"))
+        return std::max<int64_t>(1, std::min<int64_t>(bound, std::atoll(e)));
+    return bound;
 }
 
+// Walk frontier capacity per block for opening angle theta.  NB_WALK_FCAP
+// (testing) sets it directly, e.g. small enough to exercise the overflow
+// report (error bit 1); kernels stay in bounds at any value >= 1.
 int walk_fcap(double theta) {
+    if (const char* e = std::getenv("NB_WALK_FCAP")) return std::max(1, std::min(kWalkFcapMax, std::atoi(e)));
     if (!(theta < 0.5)) return kWalkFcap;
     const double r = 0.5 / std::max(theta, 1e-3);
     return (int)std::min<double>(kWalkFcapMax, std::ceil(kWalkFcap * r * r * r / 256.0) * 256.0);
@@ -1603,6 +1613,10 @@ int bh_topology(BHWorkspace* w, hipStream_t s, int64_t n, int32_t* depth, uint64
     NB_HIP(hipStreamSynchronize(s));
     LevelInfo li;
     NB_HIP(hipMemcpy(&li, w->levels.ptr, sizeof(li), hipMemcpyDeviceToHost));
+    if ((size_t)li.total > w->n_level.bytes) {  // the node pool overflowed (error bit 2)
+        set_error("Barnes-Hut topology: the tree (%d nodes) exceeded the node pool", li.total);
+        return NB_ERR_CAPACITY;
+    }
     std::vector<unsigned long long> k(n);
     std::vector<unsigned> p(n);
     std::vector<int> lf(n);
@@ -1636,6 +1650,10 @@ int bh_nodes(BHWorkspace* w, hipStream_t s, int64_t capacity, int64_t* count, in
     NB_HIP(hipStreamSynchronize(s));
     LevelInfo li;
     NB_HIP(hipMemcpy(&li, w->levels.ptr, sizeof(li), hipMemcpyDeviceToHost));
+    if ((size_t)li.total > w->n_level.bytes) {  // the node pool overflowed (error bit 2)
+        set_error("Barnes-Hut nodes: the tree (%d nodes) exceeded the node pool", li.total);
+        return NB_ERR_CAPACITY;
+    }
     *count = li.total;
     if (capacity < li.total) return NB_OK;  // caller asked for the count only
     const int64_t nn = li.total;

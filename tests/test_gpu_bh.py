@@ -317,6 +317,39 @@ def test_non_finite_input_is_reported(nb, bad):
     assert all(np.array_equal(a, b) for a, b in zip(q.acc(), fresh.acc()))
 
 
+def test_frontier_overflow_is_reported(nb, monkeypatch):
+    """A walk frontier capacity below what the input needs (NB_WALK_FCAP, a
+    testing override) sets error bit 1: the walk stays in bounds, stats()
+    raises NB_ERR_CAPACITY, and the default capacity evaluates cleanly."""
+    pl = nb.ic_plummer(65536, 3)
+    monkeypatch.setenv("NB_WALK_FCAP", "16")
+    f, _ = run_bh(nb, pl, eps=0.005, cap=8, G=1.0)
+    st = f.stats(check=False)
+    assert st["frontier_overflow"] & 1 and st["max_frontier"] == 16
+    with pytest.raises(nb.NBError) as e:
+        f.stats()
+    assert e.value.code == 4  # NB_ERR_CAPACITY
+    monkeypatch.delenv("NB_WALK_FCAP")
+    g, _ = run_bh(nb, pl, eps=0.005, cap=8, G=1.0)
+    assert g.stats()["frontier_overflow"] == 0 and g.stats()["max_frontier"] > 16
+
+
+def test_node_pool_overflow_is_reported(nb, fixtures, monkeypatch):
+    """A node pool smaller than the tree (NB_BH_NODE_CAP, a testing override)
+    sets error bit 2: nothing is written past the pool, the accelerations are
+    NaN, and stats(), topology() and nodes() report the overflow rather than
+    read past the pool."""
+    d = fixtures["p1000"]
+    monkeypatch.setenv("NB_BH_NODE_CAP", "50")
+    f, p = run_bh(nb, d, cap=8)
+    assert all(np.all(np.isnan(v)) for v in p.acc())
+    assert f.stats(check=False)["frontier_overflow"] & 2
+    for call in (f.stats, lambda: f.topology(len(d["m"])), f.nodes):
+        with pytest.raises(nb.NBError) as e:
+            call()
+        assert e.value.code == 4  # NB_ERR_CAPACITY
+
+
 def test_leapfrog_with_bh_conserves_energy(nb):
     d = nb.ic_plummer(32768, 12)
     f = nb.BarnesHutForceCalculator(0.01, 0.5, 8, "quadrupole", G=1.0)
