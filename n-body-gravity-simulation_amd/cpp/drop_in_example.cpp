@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <limits>
 #include <vector>
 
 #include "nbody.hpp"
@@ -134,6 +135,18 @@ int main(int argc, char** argv) {
             return 1;
         } catch (const nbody::Error& e) {
             std::cout << "invalid argument reported: code " << e.code << "\n";
+        }
+        // Error path: a non-finite position is found on the device during the
+        // evaluation; the vector API's device check turns it into an exception.
+        try {
+            std::vector<Particle> ps = particles;
+            ps[7].position.x = std::numeric_limits<double>::infinity();
+            ForceCalculatorBarnesHutOptimized opt(0.01, 0.5, 8);
+            opt.computeAccelerations(ps);
+            std::cerr << "expected an exception for a non-finite position\n";
+            return 1;
+        } catch (const nbody::Error& e) {
+            std::cout << "non-finite input reported: code " << e.code << "\n";
         }
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << "\n";

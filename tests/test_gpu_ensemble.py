@@ -134,6 +134,7 @@ def test_cpp_drop_in_example(O, golden, fixtures, tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "invalid argument reported" in r.stdout
+    assert "non-finite input reported: code 1" in r.stdout  # NB_ERR_INVALID from the device check
     n = 1000
     buf = np.fromfile(out, dtype=np.float64)
     assert buf.size == 3 * n + 9 * n + 9 * n + 3 * n + 3 * n
