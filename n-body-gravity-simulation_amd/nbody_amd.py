@@ -596,10 +596,16 @@ class Server:
         return ids[:c.value].tolist()
 
     def result(self, rid):
-        n = self._n.pop(rid)
+        """Final state of a finished request (collected once).  An id that is
+        not finished, unknown or already collected raises NBError."""
+        n = self._n.get(rid)
+        if n is None:  # unknown here or already collected: the library says which
+            _check(lib.nb_server_result(self._h, int(rid), *([None] * 6)))
+            raise NBError(6, f"request {rid} was not submitted to this server")
         out = {k: np.empty(n) for k in ("x", "y", "z", "vx", "vy", "vz")}
         _check(lib.nb_server_result(self._h, rid, *[out[k].ctypes.data for k in
                                                     ("x", "y", "z", "vx", "vy", "vz")]))
+        del self._n[rid]
         return out
 
     def status(self):

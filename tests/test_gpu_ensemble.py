@@ -97,6 +97,29 @@ def test_forces_and_trajectories_vs_oracle(nb, O):
         assert np.percentile(e_rel, 99) < 1e-4, k
 
 
+def test_tiny_and_ragged_systems_vs_oracle(nb, O):
+    """Systems smaller than one tile (1, 2, 3, 5, 17, 63 particles) beside
+    full ones: positions within 1e-8 of the FP64 oracle after 10 steps,
+    forces to FP32 tolerance; a lone particle feels exactly no force."""
+    sizes = [1, 2, 3, 5, 17, 63, 64, 129]
+    parts, cat = make_systems(nb, sizes, seed0=400)
+    e = new_ensemble(nb, sizes, cat)
+    e.run(10)
+    g = e.download()
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for k, p in enumerate(parts):
+        sl = slice(off[k], off[k + 1])
+        st = O.leapfrog_direct(p["m"], p["x"], p["y"], p["z"], p["vx"], p["vy"], p["vz"], 1e-3, 10,
+                               1.0, 1e-4, False)
+        for f, ref in zip(("x", "y", "z"), st[:3]):
+            assert np.abs(g[f][sl] - ref).max() < 1e-8, (sizes[k], f)
+        if sizes[k] == 1:
+            assert g["ax"][sl][0] == 0.0 and g["ay"][sl][0] == 0.0 and g["az"][sl][0] == 0.0
+        else:
+            e_rel = rel_err((g["ax"][sl], g["ay"][sl], g["az"][sl]), st[6:9])
+            assert np.max(e_rel) < 1e-4, sizes[k]
+
+
 def test_autotune_round_trip(nb, tmp_path):
     sizes = [64, 200, 900, 3000, 8192]
     _, cat = make_systems(nb, sizes)

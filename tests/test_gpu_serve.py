@@ -141,6 +141,32 @@ def test_direct_steps_per_tick_does_not_change_results(nb):
             nb.Server(direct_steps_per_tick=bad)
 
 
+def test_zero_step_request_and_result_collection(nb):
+    """A request of zero steps completes on the first tick with its initial
+    state.  Asking for a result too early is an error that keeps the request
+    collectable; a result is collected once; an unknown id is an error."""
+    sy = systems(nb)
+    srv = nb.Server(direct_threshold=32768, bh_lanes=1)
+    early = srv.submit(sy[1], 20)  # more steps than one tick takes
+    with pytest.raises(nb.NBError) as e:
+        srv.result(early)  # still pending
+    assert e.value.code == 6  # NB_ERR_STATE
+    rid = srv.submit(sy[0], 0)
+    srv.tick()
+    assert srv.poll() == [rid]
+    res = srv.result(rid)
+    for f in FIELDS:
+        assert np.array_equal(res[f], sy[0][f]), f
+    for again in (rid, 12345):
+        with pytest.raises(nb.NBError) as e:
+            srv.result(again)
+        assert e.value.code == 6
+    while srv.status()["finished_uncollected"] == 0:
+        srv.tick()
+    assert srv.poll() == [early]
+    assert srv.result(early)["x"].shape == (len(sy[1]["m"]),)
+
+
 def test_bh_lane_workspace_accounting(nb):
     """Barnes-Hut lanes keep a workspace sized for the largest request they
     have run: admission charges every lane's growth up front and never
