@@ -123,9 +123,11 @@ int nb_force_destroy(nb_force* f);
 
 /* ForceCalculator::computeAccelerations(ParticleArrays&)
  * (CUDA/include/force_calculator.hpp:25-27; CPU form CPU/include/force_calculator.hpp:24-26).
- * Writes p->ax, p->ay, p->az.  Asynchronous: for Barnes-Hut, the capacity and
- * tree-consistency checks run on the device, and nb_force_bh_stats() reports
- * their outcome (an evaluation that failed one leaves its outputs invalid). */
+ * Writes p->ax, p->ay, p->az.  Asynchronous: for Barnes-Hut, the input
+ * (finite positions and masses), capacity and tree-consistency checks run on
+ * the device, and nb_force_bh_stats() reports their outcome.  An evaluation
+ * that failed one is invalid; unless only the walk frontier overflowed, its
+ * accelerations are NaN. */
 int nb_force_compute(nb_force* f, nb_particles* p);
 
 /* ForceCalculator::name() (CPU/include/force_calculator.hpp:28). */
