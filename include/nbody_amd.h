@@ -50,19 +50,6 @@ int nb_set_device(int device);
 int nb_synchronize(void);
 /* Library version string. */
 const char* nb_version(void);
-/* Diagnostic with no counterpart in the reference: FP32 issue-rate probe on the
- * current device (16 independent v_fma_f32 chains per lane, VGPR operands, 8
- * 256-lane blocks per CU).  The library is built without packed FP32, so this,
- * not the published FP32 peak, is the arithmetic ceiling of its force kernels;
- * the benchmark reports it beside their FLOP rates.  Synchronous; *ms (may be
- * NULL) receives the timed launch's duration. */
-int nb_probe_fp32_fma(double* tflops, double* ms);
-/* Diagnostic with no counterpart in the reference: the issue cost of one
- * v_rsq_f32 in FMA slots on the current device, from the same chains with one
- * rsqrt before every 12 FMAs (a softened pair term's mix) timed against FMAs
- * alone.  A term of k other instructions and one rsqrt therefore issues at
- * most at k + 1 instructions per (k + *fma_slots) FMA slots.  Synchronous. */
-int nb_probe_fp32_rsq_cost(double* fma_slots);
 
 /* ---------------------------------------------------------- particle state */
 /* Structure-of-arrays device state, FP64 throughout.  Replaces ParticleArrays
@@ -116,18 +103,42 @@ enum nb_multipole { NB_MONOPOLE = 0, NB_QUADRUPOLE = 1 };
  * ForceCalculatorBarnesHutOptimized(double, double, int)
  * (CPU/include/force_calculator_barnes_hut_optimized.hpp:40).
  * Tree topology follows the CPU reference (root cube centred on the bbox
- * midpoint, octant = x>=c | y>=c<<1 | z>=c<<2, split when count > leaf_capacity). */
+ * midpoint, octant = x>=c | y>=c<<1 | z>=c<<2, split when count > leaf_capacity).
+ * leaf_capacity = NB_LEAF_AUTO picks the capacity per evaluation from the
+ * tuning table (nb_bh_set_tuning_file), the way the reference's leaf-capacity
+ * sweep recommends one capacity per N (CPU/tests/test3_computeLeafCapacity.cpp,
+ * CPU/documentation.md:406-418). */
+#define NB_LEAF_AUTO 0
 int nb_force_create_bh(nb_force** f, double G, double softening, double theta, int leaf_capacity,
                        int multipole);
+/* Leaf-capacity table of NB_LEAF_AUTO calculators (process-wide), as written
+ * by the leaf_capacity_sweep tool (--write-tuning): text lines
+ * "n_min leaf_capacity", n_min ascending, '#' starts a comment.  An
+ * evaluation of n particles uses the capacity of the last line with
+ * n_min <= n.  By default the table is read on first use from $NB_BH_TUNING,
+ * else from tuning/bh_leaf_capacity.txt beside the library's lib/ folder;
+ * without either file every n uses capacity 8 (the reference's default).
+ * path = NULL returns to that default.  Applies to the evaluations that
+ * follow. */
+int nb_bh_set_tuning_file(const char* path);
+/* Leaf capacity an NB_LEAF_AUTO calculator uses for n particles. */
+int nb_bh_auto_leaf_capacity(int64_t n, int* leaf_capacity);
+/* Node pool a Barnes-Hut calculator reserves for n particles at this leaf
+ * capacity: an upper bound on the tree's nodes, never more than 2^28 (the
+ * walk's node-record addressing).  A tree that needs more nodes is reported
+ * by nb_force_bh_stats (flag 2), never walked.  Host arithmetic only. */
+int64_t nb_bh_node_capacity(int64_t n, int leaf_capacity);
 int nb_force_destroy(nb_force* f);
 
 /* ForceCalculator::computeAccelerations(ParticleArrays&)
  * (CUDA/include/force_calculator.hpp:25-27; CPU form CPU/include/force_calculator.hpp:24-26).
  * Writes p->ax, p->ay, p->az.  Asynchronous: for Barnes-Hut, the input
  * (finite positions and masses), capacity and tree-consistency checks run on
- * the device, and nb_force_bh_stats() reports their outcome.  An evaluation
- * that failed one is invalid; unless only the walk frontier overflowed, its
- * accelerations are NaN. */
+ * the device, and nb_force_bh_stats() reports their outcome, including the
+ * failures of every evaluation since its last call.  An evaluation that failed
+ * one is invalid and its accelerations are NaN (after a walk-frontier
+ * overflow, those of the targets whose walk group overflowed), so a failure
+ * also shows in every later state that depends on it. */
 int nb_force_compute(nb_force* f, nb_particles* p);
 
 /* ForceCalculator::name() (CPU/include/force_calculator.hpp:28). */
@@ -169,10 +180,16 @@ typedef struct nb_bh_stats {
     double root_center[3];
     double root_size;
     int64_t n_groups;          /* walk groups (tree cells of <= 64 particles) */
+    int32_t flags_since_read;  /* OR of the error flags of every evaluation since the previous
+                                  nb_force_bh_stats call (the last one's included); reading
+                                  clears it */
+    int32_t leaf_capacity;     /* leaf capacity of the last evaluation */
 } nb_bh_stats;
-/* Fills *out and returns NB_OK, or (with *out filled) NB_ERR_INVALID if the
- * last evaluation had a non-finite input (flag 16), NB_ERR_CAPACITY for the
- * other flags, NB_ERR_STATE before any evaluation.  Synchronizes. */
+/* Fills *out and returns NB_OK, or (with *out filled) NB_ERR_CAPACITY if an
+ * evaluation since the previous call set flag 1, 2, 4 or 8 (its NaN
+ * accelerations can make later inputs non-finite), else NB_ERR_INVALID if
+ * one had a non-finite input (flag 16); NB_ERR_STATE before any evaluation.
+ * Synchronizes. */
 int nb_force_bh_stats(nb_force* f, nb_bh_stats* out);
 
 /* Topology of the last Barnes-Hut tree, per particle in ORIGINAL order: depth
@@ -255,6 +272,17 @@ int nb_ic_plummer(int64_t n, uint64_t seed, double a, double M, double G, double
 /* Uniform positions in [-1,1]^3, Gaussian velocities (sigma per axis), m = 1/n. */
 int nb_ic_uniform_gaussian(int64_t n, uint64_t seed, double sigma, double* m, double* x, double* y,
                            double* z, double* vx, double* vy, double* vz);
+/* The reference's fixture generator: one std::mt19937(seed) stream with
+ * std::uniform_real_distribution<double>, masses in [0.5, 2) and coordinates
+ * in [-100, 100).  order 0 draws m, x, y, z per particle -- the stream that
+ * wrote particles_{100,500,1000,5000,10000}.txt from seed 42, one file after
+ * the other (skip = particles drawn for the earlier files: 0, 100, 600, 1600,
+ * 6600); order 1 draws z, y, x, m (the g++ build of
+ * CPU/tests/test2_ForceBenchmark.cpp:48-57); order 2 draws x, y, z, m
+ * (CUDA/tests/test2_ForceBenchmark.cu:63-66).  Velocities of the fixtures
+ * are zero.  The first `skip` particles of the stream are drawn and dropped. */
+int nb_ic_reference_fixture(int64_t n, uint32_t seed, int order, int64_t skip, double* m,
+                            double* x, double* y, double* z);
 /* Plummer sphere (fraction 1-disk_frac of M, scale a) plus a cold exponential
  * disk (disk_frac of M, scale length rd, sech^2 thickness zd) on circular
  * orbits in the enclosed mass. */
@@ -307,8 +335,10 @@ int nb_ensemble_tuning(const nb_ensemble* e, int32_t* shape_codes, int n_buckets
  * advances every active request -- direct-method requests (n <
  * direct_threshold) together in batched launches, K leapfrog steps per tick
  * (K = 4 unless set by nb_server_set_direct_steps, and no more than any of
- * them has left), Barnes-Hut requests one step each on a pool of bh_lanes
- * calculators with their own streams, overlapping on the device.
+ * them has left), Barnes-Hut requests B steps each (B = 1 unless set by
+ * nb_server_set_bh_steps) on a pool of bh_lanes calculators with their own
+ * streams, overlapping on the device.  A request that fails finishes early
+ * with its error status (nb_server_result) without stopping the others.
  * Finished requests are downloaded and released; pending ones are admitted as
  * memory frees up.  Budget accounting: every active request is charged its
  * device state (80 B/particle; direct requests 16 B/particle more for the FP32
@@ -344,7 +374,11 @@ int nb_server_poll(nb_server* s, int64_t* ids, int capacity, int* count);
  * split a request's latency into queueing and service time.  The server
  * keeps one id per admitted request until it is polled here. */
 int nb_server_poll_admitted(nb_server* s, int64_t* ids, int capacity, int* count);
-/* Final positions/velocities of a finished request (then forgotten). */
+/* Final positions/velocities of a finished request (then forgotten).
+ * Returns the request's status: NB_OK, or the error that ended it early (a
+ * failed launch or allocation, NB_ERR_CAPACITY / NB_ERR_INVALID for a
+ * Barnes-Hut evaluation whose device checks failed, see nb_force_bh_stats);
+ * the arrays then hold the state at that point. */
 int nb_server_result(nb_server* s, int64_t id, double* x, double* y, double* z, double* vx,
                      double* vy, double* vz);
 int nb_server_get_status(const nb_server* s, nb_server_status* out);
@@ -352,6 +386,9 @@ int nb_server_get_status(const nb_server* s, nb_server_status* out);
  * finish small requests sooner while tree steps of large ones share the
  * device; results do not depend on it (each step is the same arithmetic). */
 int nb_server_set_direct_steps(nb_server* s, int steps_per_tick);
+/* Barnes-Hut request steps per tick, 1..64 (default 1).  More steps per tick
+ * mean fewer host synchronisations per step; results do not depend on it. */
+int nb_server_set_bh_steps(nb_server* s, int steps_per_tick);
 
 #ifdef __cplusplus
 }

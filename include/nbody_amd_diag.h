@@ -1,0 +1,31 @@
+/*
+ * nbody_amd_diag.h -- measurement probes of libnbody_amd.so.  They have no
+ * counterpart in the reference and are not part of the drop-in interface
+ * (include/nbody_amd.h); the benchmark uses them to state the arithmetic
+ * ceiling of this build's force kernels beside their measured FLOP rates.
+ */
+#ifndef NBODY_AMD_DIAG_H
+#define NBODY_AMD_DIAG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FP32 issue-rate probe on the current device (16 independent v_fma_f32
+ * chains per lane, VGPR operands, 8 256-lane blocks per CU).  The library is
+ * built without packed FP32, so this, not the published FP32 peak, is the
+ * arithmetic ceiling of its force kernels.  Synchronous; *ms (may be NULL)
+ * receives the timed launch's duration. */
+int nb_probe_fp32_fma(double* tflops, double* ms);
+/* The issue cost of one v_rsq_f32 in FMA slots on the current device, from
+ * the same chains with one rsqrt before every 12 FMAs (a softened pair term's
+ * mix) timed against FMAs alone.  A term of k other instructions and one
+ * rsqrt therefore issues at most at k + 1 instructions per (k + *fma_slots)
+ * FMA slots.  Synchronous. */
+int nb_probe_fp32_rsq_cost(double* fma_slots);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_AMD_DIAG_H */

@@ -26,6 +26,20 @@ static void put(std::ofstream& o, const std::vector<double>& v) {
     o.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * sizeof(double)));
 }
 
+// Host copy of device-resident state, sized from particles.num_particles the
+// way the reference's CUDA test helpers size theirs (CUDA/src/test_common.cpp:32).
+struct HostState {
+    std::vector<double> x, y, z, vx, vy, vz, ax, ay, az, m;
+};
+static HostState hostState(const ParticleArrays& particles) {
+    const size_t n = particles.num_particles;
+    HostState h;
+    for (auto* v : {&h.x, &h.y, &h.z, &h.vx, &h.vy, &h.vz, &h.ax, &h.ay, &h.az, &h.m}) v->resize(n);
+    copyParticlesToHost(particles, h.x.data(), h.y.data(), h.z.data(), h.vx.data(), h.vy.data(),
+                        h.vz.data(), h.ax.data(), h.ay.data(), h.az.data(), h.m.data());
+    return h;
+}
+
 int main(int argc, char** argv) {
     if (argc < 3) {
         std::cerr << "usage: " << argv[0] << " particles.txt out.bin\n";
@@ -83,11 +97,17 @@ int main(int argc, char** argv) {
             ForceCalculatorNaive naive(0.01);
             RK4Integrator rk;
             for (int s = 0; s < 5; ++s) rk.step(naive, d, 1e-3);
-            std::vector<double> x(N), y(N), z(N), vx(N), vy(N), vz(N), ax(N), ay(N), az(N);
-            copyParticlesToHost(d, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),
-                                ax.data(), ay.data(), az.data(), nullptr);
-            for (auto* v : {&x, &y, &z, &vx, &vy, &vz, &ax, &ay, &az}) put(out, *v);
+            const HostState h = hostState(d);
+            if (d.num_particles != N || h.m != hm) {
+                std::cerr << "device state does not round-trip\n";
+                return 1;
+            }
+            for (auto* v : {&h.x, &h.y, &h.z, &h.vx, &h.vy, &h.vz, &h.ax, &h.ay, &h.az}) put(out, *v);
             freeParticles(d);
+            if (d.num_particles != 0 || d.pos_x != nullptr) {
+                std::cerr << "freeParticles left the arrays set\n";
+                return 1;
+            }
             std::cout << rk.name() << " 5 steps done\n";
         }
         // 4. CUDA-style Barnes-Hut force.

@@ -14,14 +14,26 @@
 // prints them, followed by one JSON line per particle count with the fastest
 // configuration whose mean relative error is below --max-error.
 //
+// With --input plummer the inputs are seeded Plummer spheres (a = 1, M = 1,
+// G as the calculators use it) instead of the reference's uniform cubes.
+// --write-tuning PATH writes the leaf-capacity table that NB_LEAF_AUTO
+// calculators read (nb_bh_set_tuning_file): one "n_min leaf_capacity" line
+// per particle count, the fastest capacity under --max-error, switching at
+// the geometric midpoint between consecutive counts -- the per-N
+// recommendation table of the reference (CPU/documentation.md:406-418) in a
+// form the library consumes.
+//
 // usage: leaf_capacity_sweep [--n 100,500,1000,5000,10000] [--caps 1,2,4,8,16,32,64]
 //                            [--theta 0.5] [--experiments 5] [--iterations 100]
 //                            [--softening 0] [--monopole] [--max-error 1e-3]
+//                            [--input reference|plummer] [--write-tuning PATH]
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iomanip>
+#include <fstream>
 #include <iostream>
 #include <random>
 #include <sstream>
@@ -76,6 +88,8 @@ int main(int argc, char** argv) {
     int experiments = 5, iterations = 100;
     double softening = 0.0, max_error = 1e-3;
     int multipole = NB_QUADRUPOLE;
+    bool plummer = false;
+    const char* tuning_path = nullptr;
     for (int a = 1; a < argc; ++a) {
         const bool more = a + 1 < argc;
         if (!std::strcmp(argv[a], "--n") && more) counts = parse_list(argv[++a]);
@@ -86,6 +100,14 @@ int main(int argc, char** argv) {
         else if (!std::strcmp(argv[a], "--softening") && more) softening = std::atof(argv[++a]);
         else if (!std::strcmp(argv[a], "--max-error") && more) max_error = std::atof(argv[++a]);
         else if (!std::strcmp(argv[a], "--monopole")) multipole = NB_MONOPOLE;
+        else if (!std::strcmp(argv[a], "--input") && more) {
+            const char* v = argv[++a];
+            if (!std::strcmp(v, "plummer")) plummer = true;
+            else if (std::strcmp(v, "reference")) {
+                std::cerr << "--input must be reference or plummer\n";
+                return 2;
+            }
+        } else if (!std::strcmp(argv[a], "--write-tuning") && more) tuning_path = argv[++a];
         else {
             std::cerr << "unknown argument " << argv[a] << "\n";
             return 2;
@@ -95,6 +117,8 @@ int main(int argc, char** argv) {
         std::cerr << "experiments and iterations must be positive\n";
         return 2;
     }
+    std::sort(counts.begin(), counts.end());
+    std::vector<std::pair<int, int>> table;  // (n, recommended leaf capacity)
     try {
         for (double Nd : counts) {
             const int N = (int)Nd;
@@ -110,19 +134,28 @@ int main(int argc, char** argv) {
                     results.push_back(r);
                 }
             for (int experiment = 0; experiment < experiments; experiment++) {
-                std::mt19937 rng(42 + experiment);
-                std::uniform_real_distribution<double> position(-100.0, 100.0);
-                std::uniform_real_distribution<double> mass(0.5, 2.0);
                 std::vector<Particle> particles;
                 particles.reserve(N);
-                for (int i = 0; i < N; i++) {
-                    // The reference draws inside one emplace_back call; g++ evaluates
-                    // those arguments right to left, so the stream order is z, y, x, m.
-                    const double z = position(rng);
-                    const double y = position(rng);
-                    const double x = position(rng);
-                    const double m = mass(rng);
-                    particles.emplace_back(m, Vector3D(x, y, z), Vector3D(0, 0, 0), "Particle");
+                if (plummer) {
+                    std::vector<double> m(N), x(N), y(N), z(N), vx(N), vy(N), vz(N);
+                    nbody::check(nb_ic_plummer(N, 42 + experiment, 1.0, 1.0, constant::G, 0.0,
+                                               m.data(), x.data(), y.data(), z.data(), vx.data(),
+                                               vy.data(), vz.data()),
+                                 "nb_ic_plummer");
+                    for (int i = 0; i < N; i++)
+                        particles.emplace_back(m[i], Vector3D(x[i], y[i], z[i]), Vector3D(0, 0, 0),
+                                               "Particle");
+                } else {
+                    // The reference's inputs: std::mt19937(42 + experiment), drawn
+                    // inside one emplace_back call, which g++ evaluates right to
+                    // left: the stream order is z, y, x, m.
+                    std::vector<double> m(N), x(N), y(N), z(N);
+                    nbody::check(nb_ic_reference_fixture(N, 42 + experiment, 1, 0, m.data(),
+                                                         x.data(), y.data(), z.data()),
+                                 "nb_ic_reference_fixture");
+                    for (int i = 0; i < N; i++)
+                        particles.emplace_back(m[i], Vector3D(x[i], y[i], z[i]), Vector3D(0, 0, 0),
+                                               "Particle");
                 }
                 std::vector<Particle> reference = particles;
                 ForceCalculatorNaive naive(softening);
@@ -185,6 +218,7 @@ int main(int argc, char** argv) {
                     best = &r;
                     break;
                 }
+            if (best) table.emplace_back(N, best->leafCapacity);
             std::cout << "\n{\"n\": " << N << ", \"max_error\": " << max_error;
             if (best)
                 std::cout << ", \"theta\": " << best->theta << ", \"leaf_capacity\": "
@@ -203,6 +237,33 @@ int main(int argc, char** argv) {
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << "\n";
         return 1;
+    }
+    if (tuning_path) {
+        if (table.empty()) {
+            std::cerr << "no configuration met --max-error: no table written\n";
+            return 1;
+        }
+        std::ofstream t(tuning_path);
+        t << "# leaf capacity per particle count for NB_LEAF_AUTO Barnes-Hut calculators\n"
+          << "# command:";
+        for (int a = 0; a < argc; ++a) t << " " << argv[a];
+        t << "\n# (leaf_capacity_sweep: input " << (plummer ? "plummer" : "reference")
+          << ", theta";
+        for (double th : thetas) t << " " << th;
+        t << ", " << (multipole == NB_QUADRUPOLE ? "quadrupole" : "monopole")
+          << ", fastest with mean relative error < " << max_error << ")\n"
+          << "# n_min leaf_capacity\n";
+        for (size_t k = 0; k < table.size(); ++k) {
+            const long long nmin =
+                k == 0 ? 0
+                       : (long long)std::llround(std::sqrt((double)table[k - 1].first * table[k].first));
+            t << nmin << " " << table[k].second << "   # measured at n = " << table[k].first << "\n";
+        }
+        if (!t) {
+            std::cerr << "cannot write " << tuning_path << "\n";
+            return 1;
+        }
+        std::cout << "wrote " << tuning_path << "\n";
     }
     return 0;
 }

@@ -20,6 +20,7 @@
 // (CPU/include/common.hpp:28); setG() changes it.
 #pragma once
 
+#include <climits>
 #include <cmath>
 #include <iomanip>
 #include <iostream>
@@ -97,8 +98,28 @@ struct TimingStatistics {
     }
 };
 
-// Device state (ParticleArrays, CUDA/include/particle_data.hpp:6-43).
-using ParticleArrays = nb_particles;
+// Device state with the reference's field names (ParticleArrays,
+// CUDA/include/particle_data.hpp:6-43): FP64 device arrays owned by
+// allocateParticles / freeParticles.  The reference's Barnes-Hut scratch
+// arrays (Morton codes, sorted FP32 copies) live in the calculator's
+// workspace here, and velocity Verlet keeps its previous acceleration inside
+// the integrator, so the old_acc_* fields stay null.
+struct ParticleArrays {
+    int num_particles = 0;
+    double* pos_x = nullptr;
+    double* pos_y = nullptr;
+    double* pos_z = nullptr;
+    double* vel_x = nullptr;
+    double* vel_y = nullptr;
+    double* vel_z = nullptr;
+    double* acc_x = nullptr;
+    double* acc_y = nullptr;
+    double* acc_z = nullptr;
+    double* old_acc_x = nullptr;
+    double* old_acc_y = nullptr;
+    double* old_acc_z = nullptr;
+    double* mass = nullptr;
+};
 
 namespace nbody {
 
@@ -109,6 +130,23 @@ struct Error : std::runtime_error {
 
 inline void check(int rc, const char* where) {
     if (rc != NB_OK) throw Error(rc, std::string(where) + ": " + nb_last_error());
+}
+
+// The C ABI's view of a ParticleArrays (same device arrays, no copy).
+inline nb_particles view(const ParticleArrays& p) {
+    nb_particles v{};
+    v.n = p.num_particles;
+    v.x = p.pos_x;
+    v.y = p.pos_y;
+    v.z = p.pos_z;
+    v.vx = p.vel_x;
+    v.vy = p.vel_y;
+    v.vz = p.vel_z;
+    v.ax = p.acc_x;
+    v.ay = p.acc_y;
+    v.az = p.acc_z;
+    v.m = p.mass;
+    return v;
 }
 
 // Host <-> device staging for the CPU-style API.
@@ -164,18 +202,39 @@ struct DeviceMirror {
 
 // CUDA/include/particle_data.hpp:45-73
 inline void allocateParticles(ParticleArrays& p, size_t n) {
-    nbody::check(nb_particles_alloc(&p, (int64_t)n), "allocateParticles");
+    if (n > (size_t)INT_MAX)  // num_particles is an int, as in the reference
+        throw nbody::Error(NB_ERR_INVALID, "allocateParticles: too many particles");
+    nb_particles v{};
+    nbody::check(nb_particles_alloc(&v, (int64_t)n), "allocateParticles");
+    p = ParticleArrays{};
+    p.num_particles = (int)n;
+    p.pos_x = v.x;
+    p.pos_y = v.y;
+    p.pos_z = v.z;
+    p.vel_x = v.vx;
+    p.vel_y = v.vy;
+    p.vel_z = v.vz;
+    p.acc_x = v.ax;
+    p.acc_y = v.ay;
+    p.acc_z = v.az;
+    p.mass = v.m;
 }
-inline void freeParticles(ParticleArrays& p) { nbody::check(nb_particles_free(&p), "freeParticles"); }
+inline void freeParticles(ParticleArrays& p) {
+    nb_particles v = nbody::view(p);
+    nbody::check(nb_particles_free(&v), "freeParticles");
+    p = ParticleArrays{};
+}
 inline void copyParticlesToDevice(ParticleArrays& p, const double* x, const double* y,
                                   const double* z, const double* vx, const double* vy,
                                   const double* vz, const double* m) {
-    nbody::check(nb_particles_upload(&p, x, y, z, vx, vy, vz, m), "copyParticlesToDevice");
+    nb_particles v = nbody::view(p);
+    nbody::check(nb_particles_upload(&v, x, y, z, vx, vy, vz, m), "copyParticlesToDevice");
 }
 inline void copyParticlesToHost(const ParticleArrays& p, double* x, double* y, double* z,
                                 double* vx, double* vy, double* vz, double* ax, double* ay,
                                 double* az, double* m) {
-    nbody::check(nb_particles_download(&p, x, y, z, vx, vy, vz, ax, ay, az, m),
+    const nb_particles v = nbody::view(p);
+    nbody::check(nb_particles_download(&v, x, y, z, vx, vy, vz, ax, ay, az, m),
                  "copyParticlesToHost");
 }
 
@@ -208,12 +267,13 @@ class ForceCalculator {
 
     virtual void computeAccelerations(std::vector<Particle>& particles) {
         mirror_.upload(particles);
-        computeAccelerations(mirror_.d);
+        nbody::check(nb_force_compute(h_, &mirror_.d), "computeAccelerations");
         mirror_.download(particles, false);
         checkDevice();
     }
     virtual void computeAccelerations(ParticleArrays& particles) {
-        nbody::check(nb_force_compute(h_, &particles), "computeAccelerations");
+        nb_particles v = nbody::view(particles);
+        nbody::check(nb_force_compute(h_, &v), "computeAccelerations");
     }
     // Throws if a check that ran on the device failed in the last evaluation
     // (the synchronous std::vector calls run it after downloading).
@@ -348,16 +408,18 @@ class TimeIntegrator {
     virtual void step(std::vector<Particle>& particles, ForceCalculator& calculator, double dt) {
         mirror_.upload(particles);
         nb_integrator_reset(h_);  // host state may have changed since the last call
-        step(calculator, mirror_.d, dt);
+        nbody::check(nb_integrator_step(h_, calculator.handle(), &mirror_.d, dt), "step");
         mirror_.download(particles, true);
         calculator.checkDevice();
     }
     virtual void step(ForceCalculator& calculator, ParticleArrays& particles, double dt) {
-        nbody::check(nb_integrator_step(h_, calculator.handle(), &particles, dt), "step");
+        nb_particles v = nbody::view(particles);
+        nbody::check(nb_integrator_step(h_, calculator.handle(), &v, dt), "step");
     }
     // Several steps without host round trips (device state).
     void run(ForceCalculator& calculator, ParticleArrays& particles, double dt, int steps) {
-        nbody::check(nb_integrator_run(h_, calculator.handle(), &particles, dt, steps), "run");
+        nb_particles v = nbody::view(particles);
+        nbody::check(nb_integrator_run(h_, calculator.handle(), &v, dt, steps), "run");
     }
     virtual std::string name() const { return nb_integrator_name(h_); }
 };
@@ -459,7 +521,8 @@ class OrbitCsvWriter {
                 "total_energy,angular_momentum_x,angular_momentum_y,angular_momentum_z,"
                 "center_of_mass_x,center_of_mass_y,center_of_mass_z\n";
     }
-    void row(const ParticleArrays& p) {
+    void row(const ParticleArrays& pa) {
+        const nb_particles p = view(pa);
         std::vector<double> x(p.n), y(p.n), z(p.n);
         check(nb_particles_download(&p, x.data(), y.data(), z.data(), nullptr, nullptr, nullptr,
                                     nullptr, nullptr, nullptr, nullptr),

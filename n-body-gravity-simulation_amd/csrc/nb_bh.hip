@@ -44,6 +44,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <cstddef>
 #include <cstdlib>
 
 #include "nb_internal.hpp"
@@ -74,6 +75,7 @@ constexpr int kWalkFcap = 2048;
 constexpr int kWalkFcapMax = 65536;
 constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
 constexpr int kGroup = 64;      // walk group: cells of at most 64 particles
+constexpr int64_t kMaxNodes = (int64_t)1 << 28;  // node pool limit (32-bit record offsets in the walk)
 // Per-block count columns: 22 levels, group starts, leaf starts, oversize leaves.
 // Walk groups are handed out by size class, largest first (a group's cost
 // grows with its size: one interaction list evaluated for all its targets),
@@ -103,11 +105,13 @@ struct LevelInfo {
 struct Counters {
     unsigned long long pp, pc;
     int max_frontier;
-    int overflow;   // bit 0: walk frontier, bit 1: node capacity, bit 2: malformed tree,
-                    // bit 3: sort look-back wait limit
+    int overflow;   // this evaluation: bit 0 walk frontier, bit 1 node capacity, bit 2
+                    // malformed tree, bit 3 sort look-back wait limit, bit 4 non-finite input
     int oversize;   // leaves at the key-depth limit with > cap particles
     int n_leaves;
     int next_group;  // work counter of the persistent walk
+    int sticky;      // OR of `overflow` over every evaluation since the flags were last read
+                     // (not reset per evaluation; cleared by the host when it reads them)
 };
 
 // Radix sort state: the digit counts of all passes (filled in one pass over
@@ -171,6 +175,12 @@ __device__ __forceinline__ void reset_counters(Counters* c) {
     c->oversize = 0;
     c->n_leaves = 0;
     c->next_group = 0;
+}
+
+// Records error bits for this evaluation and in the sticky word.
+__device__ __forceinline__ void raise_flag(Counters* c, int bits) {
+    atomicOr(&c->overflow, bits);
+    atomicOr(&c->sticky, bits);
 }
 
 // ------------------------------------------------------------------ bbox
@@ -286,7 +296,7 @@ __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__
     // 16): the up-sweep and walk skip it.  One report per wave.
     const bool bad = !(isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(pm));
     const unsigned long long bb = __ballot(bad);
-    if (bb && lane_id() == __ffsll((long long)bb) - 1) atomicOr(&ctr->overflow, kInputBad);
+    if (bb && lane_id() == __ffsll((long long)bb) - 1) raise_flag(ctr, kInputBad);
     unsigned long long k = 0;
 #pragma unroll
     for (int l = 0; l < kMaxDigits; ++l) {
@@ -523,7 +533,7 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
                 tt -= used;
                 if (stall && !done) {
                     if (++spins > kLookbackSpins) {
-                        atomicOr(&ctr->overflow, 8);
+                        raise_flag(ctr, 8);
                         sbad = 1;
                         break;
                     }
@@ -701,19 +711,25 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
         blkcnt[t * nblk + blockIdx.x] = wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
 }
 
-// Level starts/counts, group and leaf totals from the column sums.
+// Level starts/counts, group and leaf totals from the column sums.  The node
+// total is summed in 64 bits: 22 levels of up to n nodes each can pass 2^31.
+// A tree larger than the pool (which is never above 2^28 nodes, the range of
+// the walk's 32-bit record offsets) sets error bit 2; its total is stored
+// clamped so every later stage sees total > node_cap and writes nothing.
 __global__ void k_level_info(int n, const int* __restrict__ totals, LevelInfo* __restrict__ li,
                              int node_cap, int* __restrict__ gstart, Counters* __restrict__ ctr) {
     if (threadIdx.x != 0) return;
-    int run = 0, ml = 0;
+    long long run = 0;
+    int ml = 0;
     for (int l = 0; l < kLevels; ++l) {
-        li->start[l] = run;
+        li->start[l] = (int)min(run, (long long)INT_MAX);
         li->count[l] = totals[l];
         run += totals[l];
         if (totals[l] > 0) ml = l;
     }
-    li->start[kLevels] = run;
-    li->total = run;
+    const int total = (int)min(run, (long long)INT_MAX);
+    li->start[kLevels] = total;
+    li->total = total;
     li->max_level = ml;
     li->n_groups = totals[kColGroup];
     int gs = 0;
@@ -724,7 +740,7 @@ __global__ void k_level_info(int n, const int* __restrict__ totals, LevelInfo* _
     gstart[totals[kColGroup]] = n;
     ctr->n_leaves = totals[kColLeaf];
     ctr->oversize = totals[kColOver];
-    if (run > node_cap) atomicOr(&ctr->overflow, 2);
+    if (run > (long long)node_cap) raise_flag(ctr, 2);
 }
 
 // Writes the node records: particle range, level, child range; and the leaf
@@ -807,7 +823,7 @@ __global__ __launch_bounds__(kBlk) void k_emit(
             const bool starts = shi < l;
             const int id = lstart[l] + rl + (starts ? 1 : 0) - 1;
             if (id < lstart[l] || id >= lstart[l + 1]) {
-                atomicOr(&ctr->overflow, 4);
+                raise_flag(ctr, 4);
             } else {
                 if (starts) {
                     pstart[id] = i;
@@ -843,14 +859,14 @@ __device__ __forceinline__ void node_moments(int id, int n, int total, const Roo
     if constexpr (kLeaf) {
         const int a = pstart[id], b = pend[id];
         if (a < 0 || b > n || a >= b) {
-            atomicOr(&ctr->overflow, 4);
+            raise_flag(ctr, 4);
             link[id] = make_int2(~0, 0);
             return;
         }
     } else {
         const int c1 = cend[id];
         if (c0 <= id || c1 > total || c0 >= c1 || c1 - c0 > 8) {
-            atomicOr(&ctr->overflow, 4);
+            raise_flag(ctr, 4);
             link[id] = make_int2(~0, 0);
             return;
         }
@@ -1105,6 +1121,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
 
         float fxA = 0.f, fyA = 0.f, fzA = 0.f, fxB = 0.f, fyB = 0.f, fzB = 0.f;
         int ncl = 0, npl = 0, nnext = 0, nb = 0;
+        bool gov = false;  // this group's frontier overflowed: its sums are incomplete
         unsigned long long npp = 0, npc = 0;
         int2* nxtg = fr0;
 
@@ -1198,7 +1215,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                 if (pos < 64) fl[nb][pos] = lk;
                 else if (pos < fcap) nxtg[pos] = lk;
             }
-            if (nnext + no > fcap) ovf = true;
+            if (nnext + no > fcap) gov = true;
             nnext = min(nnext + no, fcap);
 
             // Accepted cells: staged in LDS and evaluated cfull at a time
@@ -1338,16 +1355,20 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             }
             __syncthreads();
         }
+        // A group whose frontier overflowed skipped cells: its accelerations
+        // are NaN (error bit 1), never silently incomplete.
+        ovf = ovf || gov;
+        const double gs = gov ? __builtin_nan("") : G;
         if (lead) {
             const unsigned j = perm[kA];
-            ax[j] = G * (double)fxA;
-            ay[j] = G * (double)fyA;
-            az[j] = G * (double)fzA;
+            ax[j] = gs * (double)fxA;
+            ay[j] = gs * (double)fyA;
+            az[j] = gs * (double)fzA;
             if (hasB) {
                 const unsigned jb = perm[kB];
-                ax[jb] = G * (double)fxB;
-                ay[jb] = G * (double)fyB;
-                az[jb] = G * (double)fzB;
+                ax[jb] = gs * (double)fxB;
+                ay[jb] = gs * (double)fyB;
+                az[jb] = gs * (double)fzB;
             }
         }
         tot_pp += npp * (unsigned long long)nact;
@@ -1358,16 +1379,22 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
         atomicAdd(&ctr->pp, tot_pp);
         atomicAdd(&ctr->pc, tot_pc);
         atomicMax(&ctr->max_frontier, maxf);
-        if (ovf) atomicOr(&ctr->overflow, 1);
+        if (ovf) raise_flag(ctr, 1);
     }
 }
 
 // ------------------------------------------------------------------ host
 // Node pool size: an upper bound on the tree's nodes (every particle a leaf
-// on a chain of up to 21 levels per cap + 1 particles).  NB_BH_NODE_CAP
-// (testing) can only lower it, to exercise the overflow report (error bit 2).
-static int64_t node_capacity(int64_t n, int cap) {
-    const int64_t bound = n + (int64_t)kMaxDigits * ((n + cap) / (cap + 1)) + 1;
+// on a chain of up to 21 levels per cap + 1 particles), never above
+// kMaxNodes: the walk addresses node records with 32-bit byte offsets
+// (16-byte records), so node ids must stay below 2^28.  A tree with more
+// nodes than the pool is reported (error bit 2), never walked.
+// NB_BH_NODE_CAP (testing) can only lower it, to exercise that report.
+int64_t node_capacity(int64_t n, int cap) {
+    if (n <= 0) return 0;
+    cap = std::max(1, cap);
+    const int64_t bound =
+        std::min<int64_t>(kMaxNodes, n + (int64_t)kMaxDigits * ((n + cap) / (cap + 1)) + 1);
     if (const char* e = std::getenv("NB_BH_NODE_CAP"))
         return std::max<int64_t>(1, std::min<int64_t>(bound, std::atoll(e)));
     return bound;
@@ -1420,15 +1447,16 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     const int nblk = grid_for(n, kBlk);
     const int nsort = grid_for(n, kOsTile);
     const int nbb = std::min(1024, nblk);
-    const int64_t ncap = node_capacity(n, leaf_cap);
-    if (ncap > INT_MAX / 2) {
-        set_error("Barnes-Hut: node capacity overflow");
-        return NB_ERR_INVALID;
-    }
+    const int64_t ncap = node_capacity(n, leaf_cap);  // <= kMaxNodes
 
     NB_TRY(w->root.reserve(sizeof(RootInfo)));
     NB_TRY(w->levels.reserve(sizeof(LevelInfo)));
-    NB_TRY(w->counters.reserve(sizeof(Counters)));
+    {
+        const bool fresh = w->counters.ptr == nullptr;
+        NB_TRY(w->counters.reserve(sizeof(Counters)));
+        // The sticky flag word outlives evaluations: it starts at zero.
+        if (fresh) NB_HIP(hipMemsetAsync(w->counters.ptr, 0, sizeof(Counters), s));
+    }
     NB_TRY(w->bbox_part.reserve((size_t)nbb * 6 * sizeof(double)));
     NB_TRY(w->keys0.reserve((size_t)n * 8));
     NB_TRY(w->keys1.reserve((size_t)n * 8));
@@ -1594,12 +1622,37 @@ int bh_stats(BHWorkspace* w, hipStream_t s, nb_bh_stats* out) {
     for (int a = 0; a < 3; ++a) out->root_center[a] = r.center[a];
     out->root_size = r.size;
     out->n_groups = li.n_groups;
-    if (c.overflow) {
-        set_error("Barnes-Hut: invalid evaluation (flags %d: 1=walk frontier, 2=node pool, "
-                  "4=tree consistency, 8=sort look-back wait, 16=non-finite position or mass)",
-                  c.overflow);
-        return (c.overflow & kInputBad) ? NB_ERR_INVALID : NB_ERR_CAPACITY;
+    const int all = c.overflow | c.sticky;
+    out->flags_since_read = all;
+    out->leaf_capacity = w->last_cap;
+    if (c.sticky) {  // reading the flags clears the sticky word
+        const int zero = 0;
+        NB_HIP(hipMemcpy(reinterpret_cast<char*>(w->counters.ptr) + offsetof(Counters, sticky),
+                         &zero, sizeof(zero), hipMemcpyHostToDevice));
     }
+    if (all) {
+        set_error("Barnes-Hut: invalid evaluation (last evaluation's flags %d, flags since the "
+                  "last read %d: 1=walk frontier, 2=node pool, 4=tree consistency, 8=sort "
+                  "look-back wait, 16=non-finite position or mass)",
+                  c.overflow, all);
+        // A capacity or consistency failure NaNs accelerations, which can reach
+        // later evaluations as non-finite input: it is the cause to report.
+        return (all & ~kInputBad) ? NB_ERR_CAPACITY : NB_ERR_INVALID;
+    }
+    return NB_OK;
+}
+
+// ORs the current evaluation's error flags into *dst (stream-ordered after
+// the evaluation), so a caller that shares one calculator between several
+// systems can attribute a failure to the system that caused it.
+__global__ void k_flags_to(const Counters* __restrict__ ctr, int* __restrict__ dst) {
+    if (threadIdx.x == 0) *dst |= ctr->overflow;
+}
+
+int bh_flags_to(BHWorkspace* w, int* dst, hipStream_t s) {
+    if (!w->counters.ptr) return NB_OK;
+    hipLaunchKernelGGL(k_flags_to, dim3(1), dim3(64), 0, s, w->counters.as<Counters>(), dst);
+    NB_LAUNCH_CHECK();
     return NB_OK;
 }
 
@@ -1686,3 +1739,7 @@ int bh_nodes(BHWorkspace* w, hipStream_t s, int64_t capacity, int64_t* count, in
 }
 
 }  // namespace nb
+
+extern "C" int64_t nb_bh_node_capacity(int64_t n, int leaf_capacity) {
+    return nb::node_capacity(n, leaf_capacity);
+}

@@ -66,7 +66,8 @@ __global__ __launch_bounds__(kDiagBlock) void k_diag_partial(
 }
 
 // Per-target potential sums phi_i = sum_{j != i} m_j / r_ij (and softened),
-// then a compensated block reduction of m_i * phi_i.
+// each Neumaier-compensated over its N-1 terms, then a compensated block
+// reduction of m_i * phi_i.
 __global__ __launch_bounds__(kDiagBlock) void k_diag_potential(
     int n, const double* __restrict__ m, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, double soft2,
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(kDiagBlock) void k_diag_potential(
         const int i = i0 + t;
         const bool valid = i < n;
         const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
-        double phu = 0.0, phs = 0.0;
+        Neu phu{0.0, 0.0}, phs{0.0, 0.0};
         for (int j0 = 0; j0 < n; j0 += kDiagBlock) {
             const int j = j0 + t;
             if (j < n) {
@@ -95,15 +96,15 @@ __global__ __launch_bounds__(kDiagBlock) void k_diag_potential(
                     if (j0 + k == i) continue;
                     const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
                     const double r2 = dx * dx + dy * dy + dz * dz;
-                    phu += sm[k] / sqrt(r2);
-                    phs += sm[k] / sqrt(r2 + soft2);
+                    neu_add(phu, sm[k] / sqrt(r2));
+                    neu_add(phs, sm[k] / sqrt(r2 + soft2));
                 }
             }
             __syncthreads();
         }
         if (valid) {
-            neu_add(pu, m[i] * phu);
-            neu_add(ps, m[i] * phs);
+            neu_add(pu, m[i] * (phu.s + phu.c));
+            neu_add(ps, m[i] * (phs.s + phs.c));
         }
     }
     for (int q = 0; q < 2; ++q) {

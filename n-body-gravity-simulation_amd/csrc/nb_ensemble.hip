@@ -415,6 +415,7 @@ int nb_ensemble_run(nb_ensemble* e, int steps, int use_graph) {
 
 int nb_ensemble_run_unbatched(nb_ensemble* e, int steps) {
     if (!e || steps < 0) return NB_ERR_INVALID;
+    if (steps == 0) return NB_OK;  // as nb_ensemble_run: no step, no closing kick
     NB_TRY(ens_prime(e));
     const EnsItem* items = e->d_items_by_sys.as<EnsItem>();
     for (int k = 0; k < steps; ++k) {

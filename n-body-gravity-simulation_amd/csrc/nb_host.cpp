@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -242,6 +243,46 @@ int nb_ic_plummer_disk(int64_t n, uint64_t seed, double a, double M, double G, d
         }
     }
     to_com_frame(n, m, x, y, z, vx, vy, vz);
+    return NB_OK;
+}
+
+// The reference's fixture stream: std::mt19937 with libstdc++'s
+// uniform_real_distribution<double> (the same engine and distribution the
+// reference's generator and test mains use, so the draws are the same bits).
+int nb_ic_reference_fixture(int64_t n, uint32_t seed, int order, int64_t skip, double* m,
+                            double* x, double* y, double* z) {
+    if (n < 0 || skip < 0 || order < 0 || order > 2 || (n > 0 && (!m || !x || !y || !z))) {
+        nb::set_error("nb_ic_reference_fixture: invalid arguments (n=%lld order=%d skip=%lld)",
+                      (long long)n, order, (long long)skip);
+        return NB_ERR_INVALID;
+    }
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> pos(-100.0, 100.0);
+    std::uniform_real_distribution<double> mass(0.5, 2.0);
+    double dm, dx, dy, dz;
+    auto draw = [&](double& pm, double& px, double& py, double& pz) {
+        switch (order) {
+            case 0:  // m, x, y, z
+                pm = mass(rng);
+                px = pos(rng);
+                py = pos(rng);
+                pz = pos(rng);
+                break;
+            case 1:  // z, y, x, m
+                pz = pos(rng);
+                py = pos(rng);
+                px = pos(rng);
+                pm = mass(rng);
+                break;
+            default:  // x, y, z, m
+                px = pos(rng);
+                py = pos(rng);
+                pz = pos(rng);
+                pm = mass(rng);
+        }
+    };
+    for (int64_t i = 0; i < skip; ++i) draw(dm, dx, dy, dz);
+    for (int64_t i = 0; i < n; ++i) draw(m[i], x[i], y[i], z[i]);
     return NB_OK;
 }
 

@@ -82,6 +82,11 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
             int64_t n, const double* m, const double* x, const double* y, const double* z,
             double* ax, double* ay, double* az, hipStream_t s, const hipEvent_t* stage_ev);
 int bh_stats(BHWorkspace* w, hipStream_t s, nb_bh_stats* out);
+// Launches a one-thread kernel on `s` that ORs the error flags of the
+// evaluation just enqueued into the device int *dst.
+int bh_flags_to(BHWorkspace* w, int* dst, hipStream_t s);
+// Node pool size for n particles at a leaf capacity (at most 2^28).
+int64_t node_capacity(int64_t n, int cap);
 int bh_topology(BHWorkspace* w, hipStream_t s, int64_t n, int32_t* depth, uint64_t* path,
                 uint64_t* keys, uint32_t* perm);
 int bh_nodes(BHWorkspace* w, hipStream_t s, int64_t capacity, int64_t* count, int32_t* level,
@@ -116,6 +121,9 @@ struct nb_force {
     double t_build_coarse = 0;  // tree-build time of timing == 2 calls
     int64_t timed_calls = 0;
     int64_t calls = 0;
+    // Barnes-Hut only: when set, every evaluation ORs its error flags into
+    // this device int (the server points it at the request being stepped).
+    int* err_sink = nullptr;
 };
 
 namespace nb {
@@ -124,6 +132,8 @@ namespace nb {
 constexpr int kStages = 7;
 // Resolves pending timing events into f->t_stage (synchronizes f->stream).
 int resolve_timing(nb_force* f);
+// Leaf capacity of an NB_LEAF_AUTO Barnes-Hut evaluation of n particles.
+int auto_leaf_capacity(int64_t n, int* cap);
 // Force evaluation on arbitrary position arrays (used by the integrators for
 // intermediate stages).
 int force_eval(nb_force* f, int64_t n, const double* m, const double* x, const double* y,

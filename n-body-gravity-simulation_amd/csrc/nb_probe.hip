@@ -8,6 +8,7 @@
 // that issue stream: the same chains with one rsqrt before every 12 FMAs
 // (the mix of a softened particle-particle term), timed against FMA alone.
 // The benchmark reports the force kernels' rates beside both.
+#include "../../include/nbody_amd_diag.h"
 #include "nb_internal.hpp"
 
 namespace nb {

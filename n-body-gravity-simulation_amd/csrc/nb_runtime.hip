@@ -1,8 +1,15 @@
 // nb_runtime.hip -- error reporting, device selection, particle state and the
 // force-calculator handles of the C ABI.
+#include <dlfcn.h>
+
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <utility>
 
 #include "nb_internal.hpp"
 
@@ -69,6 +76,93 @@ int resolve_timing(nb_force* f) {
     return NB_OK;
 }
 
+// ------------------------------------------------------- leaf-capacity table
+// (n_min, leaf capacity) rows, n_min ascending: an NB_LEAF_AUTO calculator
+// evaluating n particles uses the capacity of the last row with n_min <= n.
+// The table is read from the file named by nb_bh_set_tuning_file, else from
+// $NB_BH_TUNING, else from tuning/bh_leaf_capacity.txt beside the library's
+// lib/ folder (written by the leaf_capacity_sweep tool); without any of them
+// every n uses capacity 8, the reference's default.
+namespace {
+std::mutex g_tune_mu;
+std::vector<std::pair<int64_t, int>> g_tune;
+bool g_tune_loaded = false;
+std::string g_tune_source = "built-in";
+
+int parse_tuning(const std::string& path, std::vector<std::pair<int64_t, int>>& out) {
+    std::ifstream f(path);
+    if (!f) {
+        set_error("leaf-capacity table: cannot open %s", path.c_str());
+        return NB_ERR_INVALID;
+    }
+    out.clear();
+    std::string line;
+    int ln = 0;
+    while (std::getline(f, line)) {
+        ++ln;
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line.resize(h);
+        std::istringstream ss(line);
+        long long nmin;
+        int cap;
+        if (!(ss >> nmin)) continue;  // blank or comment line
+        std::string rest;
+        if (!(ss >> cap) || (ss >> rest) || nmin < 0 || cap < 1 || cap > 64 ||
+            (!out.empty() && nmin <= out.back().first)) {
+            set_error("leaf-capacity table %s: line %d is malformed (want \"n_min cap\", n_min "
+                      "ascending, 1 <= cap <= 64)", path.c_str(), ln);
+            return NB_ERR_INVALID;
+        }
+        out.emplace_back((int64_t)nmin, cap);
+    }
+    if (out.empty()) {
+        set_error("leaf-capacity table %s has no rows", path.c_str());
+        return NB_ERR_INVALID;
+    }
+    return NB_OK;
+}
+
+std::string default_tuning_path() {
+    if (const char* e = std::getenv("NB_BH_TUNING")) return e;
+    Dl_info info{};
+    if (!dladdr(reinterpret_cast<void*>(&nb_version), &info) || !info.dli_fname) return "";
+    std::string lib = info.dli_fname;  // .../lib/libnbody_amd.so
+    const size_t a = lib.rfind('/');
+    if (a == std::string::npos) return "";
+    const std::string dir = lib.substr(0, a);
+    const size_t b = dir.rfind('/');
+    return (b == std::string::npos ? std::string(".") : dir.substr(0, b)) +
+           "/tuning/bh_leaf_capacity.txt";
+}
+
+// With g_tune_mu held.  A missing default file means the built-in table; a
+// malformed one is an error (reported by the evaluation that needed it).
+int load_default_tuning() {
+    std::vector<std::pair<int64_t, int>> t;
+    const std::string path = default_tuning_path();
+    if (!path.empty() && std::ifstream(path)) {
+        NB_TRY(parse_tuning(path, t));
+        g_tune = std::move(t);
+        g_tune_source = path;
+    } else {
+        g_tune = {{0, 8}};
+        g_tune_source = "built-in";
+    }
+    g_tune_loaded = true;
+    return NB_OK;
+}
+}  // namespace
+
+int auto_leaf_capacity(int64_t n, int* cap) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tune_loaded) NB_TRY(load_default_tuning());
+    int c = g_tune.front().second;
+    for (const auto& r : g_tune)
+        if (r.first <= n) c = r.second;
+    *cap = c;
+    return NB_OK;
+}
+
 int force_eval(nb_force* f, int64_t n, const double* m, const double* x, const double* y,
                const double* z, double* ax, double* ay, double* az) {
     if (n <= 0) return NB_OK;
@@ -97,8 +191,11 @@ int force_eval(nb_force* f, int64_t n, const double* m, const double* x, const d
                            f->stream));
         if (ev) NB_HIP(hipEventRecord(ev[kStages], f->stream));
     } else {
-        NB_TRY(bh_eval(f->bh, f->G, f->soft2, f->theta, f->leaf_cap, f->multipole, n, m, x, y, z,
-                       ax, ay, az, f->stream, ev));
+        int cap = f->leaf_cap;
+        if (cap == NB_LEAF_AUTO) NB_TRY(auto_leaf_capacity(n, &cap));
+        NB_TRY(bh_eval(f->bh, f->G, f->soft2, f->theta, cap, f->multipole, n, m, x, y, z, ax, ay,
+                       az, f->stream, ev));
+        if (f->err_sink) NB_TRY(bh_flags_to(f->bh, f->err_sink, f->stream));
     }
     f->calls++;
     return NB_OK;
@@ -239,7 +336,7 @@ int nb_force_create_direct(nb_force** out, int precision, double G, double softe
 
 int nb_force_create_bh(nb_force** out, double G, double softening, double theta, int leaf_capacity,
                        int multipole) {
-    if (!out || !(theta >= 0.0) || leaf_capacity < 1 || leaf_capacity > 64 ||
+    if (!out || !(theta >= 0.0) || leaf_capacity < NB_LEAF_AUTO || leaf_capacity > 64 ||
         (multipole != NB_MONOPOLE && multipole != NB_QUADRUPOLE) || !std::isfinite(G) ||
         !std::isfinite(softening)) {
         set_error("nb_force_create_bh: invalid arguments (theta=%g leaf_capacity=%d multipole=%d)",
@@ -263,6 +360,28 @@ int nb_force_create_bh(nb_force** out, double G, double softening, double theta,
     }
     *out = f;
     return NB_OK;
+}
+
+int nb_bh_set_tuning_file(const char* path) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!path) {
+        g_tune_loaded = false;  // the default table is read again on next use
+        return NB_OK;
+    }
+    std::vector<std::pair<int64_t, int>> t;
+    NB_TRY(parse_tuning(path, t));
+    g_tune = std::move(t);
+    g_tune_source = path;
+    g_tune_loaded = true;
+    return NB_OK;
+}
+
+int nb_bh_auto_leaf_capacity(int64_t n, int* leaf_capacity) {
+    if (!leaf_capacity || n < 0) {
+        set_error("nb_bh_auto_leaf_capacity: invalid arguments");
+        return NB_ERR_INVALID;
+    }
+    return auto_leaf_capacity(n, leaf_capacity);
 }
 
 int nb_force_destroy(nb_force* f) {

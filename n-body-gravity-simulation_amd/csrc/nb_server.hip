@@ -2,10 +2,11 @@
 //
 // Requests (one N-body system each, with a step count) are admitted against
 // an HBM budget and keep their state resident on the device until they
-// finish.  Every tick advances each Barnes-Hut request by one leapfrog step
-// and each direct request by K steps, K = min(the direct steps per tick, 4
-// unless set, the fewest steps any direct request has left), so that cheap
-// requests are not paced by the tree steps in the same tick:
+// finish.  Every tick advances each Barnes-Hut request by up to B leapfrog
+// steps (B = 1 unless set, no more than it has left) and each direct request
+// by K steps, K = min(the direct steps per tick, 4 unless set, the fewest
+// steps any direct request has left), so that cheap requests are not paced by
+// the tree steps in the same tick:
 //   * direct-method requests (N below a threshold) are stepped together: per
 //     step one batched kick-drift launch, one persistent force launch over all
 //     of them (work items (request, target tile, shape) pulled from an atomic
@@ -15,7 +16,11 @@
 //     walks of different requests overlap on the device with each other and
 //     with the direct batch.
 // Finished requests are downloaded, their memory is released, and pending
-// requests are admitted as the budget allows.  A request's arithmetic does
+// requests are admitted as the budget allows.  A request whose step fails
+// (a launch or allocation error, or a Barnes-Hut evaluation whose device
+// checks failed -- each BH evaluation ORs its error flags into the request's
+// own flag word, read at the tick's synchronisation) finishes at once with
+// that status; the other requests of the tick are not affected.  A request's arithmetic does
 // not depend on what else is running: a direct request's force uses the same
 // kernel and tile shape whatever the batch, a BH request runs its own
 // deterministic pipeline.
@@ -25,6 +30,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "nb_internal.hpp"
@@ -34,6 +40,7 @@ namespace nb {
 constexpr int kSrvBlock = 256;
 constexpr int kDirectStepsPerTick = 4;  // default direct-request steps per tick (fewer near their end)
 constexpr int kDirectStepsPerTickMax = 64;
+constexpr int kBHStepsPerTick = 1;  // default Barnes-Hut request steps per tick
 
 struct SrvSys {
     const float4* pack;
@@ -167,6 +174,9 @@ struct Request {
     bool have_acc = false;
     int lane = -1;
     int done = 0;
+    int stepped = 0;      // steps launched in the current tick
+    int status = NB_OK;   // a failed request finishes early with this status
+    std::string message;  // and this message (nb_server_result reports both)
 };
 
 // Device bytes a request holds while it is active: its FP64 state (10
@@ -198,6 +208,8 @@ struct nb_server {
     std::vector<int64_t> completed_since_poll;
     std::vector<int64_t> admitted_since_poll;
     int direct_steps = kDirectStepsPerTick;  // direct-request steps per tick
+    int bh_steps = kBHStepsPerTick;          // Barnes-Hut request steps per tick
+    DevBuf d_flags;  // one error-flag word per Barnes-Hut request stepped in a tick
     // Two launch slots (priming batch, stepping batch) per tick, each with its
     // own device tables and the host staging they are copied from (kept alive
     // until the tick's synchronisation).
@@ -396,54 +408,93 @@ int nb_server_submit(nb_server* s, const nb_request* req, int64_t* id) {
     return NB_OK;
 }
 
+namespace {
+void fail_request(Request* r, int status) {
+    r->status = status;
+    r->message = nb_last_error();
+}
+}  // namespace
+
 int nb_server_tick(nb_server* s, int* n_completed) {
     if (!s) return NB_ERR_INVALID;
     NB_TRY(admit(s));
-    std::vector<Request*> direct, fresh;
-    int lane_rr = 0;
+    std::vector<Request*> direct, fresh, bh;
     for (auto& up : s->active) {
         Request* r = up.get();
-        if (r->done >= r->desc.steps) continue;
+        r->stepped = 0;
+        if (r->done >= r->desc.steps || r->status != NB_OK) continue;
         if (r->bh) {
-            const int lane = lane_rr++ % s->n_lanes;
-            while ((int)s->lanes.size() <= lane) {
-                nb_force* f = nullptr;
-                NB_TRY(nb_force_create_bh(&f, r->desc.G, r->desc.softening, r->desc.theta,
-                                          s->leaf_cap, NB_QUADRUPOLE));
-                s->lanes.push_back(f);  // its workspace is charged at admission
-            }
-            nb_force* f = s->lanes[lane];
-            // A lane is a workspace + stream; the physics parameters are the request's.
-            f->G = r->desc.G;
-            f->softening = r->desc.softening;
-            f->soft2 = r->desc.softening * r->desc.softening;
-            f->theta = r->desc.theta;
-            NB_TRY(nb_integrator_step(r->integ, f, &r->p, r->desc.dt));
+            bh.push_back(r);
         } else {
             direct.push_back(r);
             if (!r->have_acc) fresh.push_back(r);
         }
     }
+    // Every Barnes-Hut evaluation of request k ORs its device error flags
+    // into word k, read back at the tick's synchronisation.
+    if (!bh.empty()) {
+        NB_TRY(s->d_flags.reserve(bh.size() * sizeof(int)));
+        NB_HIP(hipMemset(s->d_flags.ptr, 0, bh.size() * sizeof(int)));
+    }
+    for (size_t k = 0; k < bh.size(); ++k) {
+        Request* r = bh[k];
+        const int lane = (int)(k % (size_t)s->n_lanes);
+        while ((int)s->lanes.size() <= lane) {
+            nb_force* f = nullptr;
+            NB_TRY(nb_force_create_bh(&f, r->desc.G, r->desc.softening, r->desc.theta,
+                                      s->leaf_cap, NB_QUADRUPOLE));
+            s->lanes.push_back(f);  // its workspace is charged at admission
+        }
+        nb_force* f = s->lanes[lane];
+        // A lane is a workspace + stream; the physics parameters are the request's.
+        f->G = r->desc.G;
+        f->softening = r->desc.softening;
+        f->soft2 = r->desc.softening * r->desc.softening;
+        f->theta = r->desc.theta;
+        f->err_sink = s->d_flags.as<int>() + k;
+        const int nst = std::min(s->bh_steps, r->desc.steps - r->done);
+        const int rc = nb_integrator_run(r->integ, f, &r->p, r->desc.dt, nst);
+        f->err_sink = nullptr;
+        if (rc == NB_OK) r->stepped = nst;
+        else fail_request(r, rc);  // only this request fails; the tick goes on
+    }
     int kdir = s->direct_steps;
     for (Request* r : direct) kdir = std::min(kdir, r->desc.steps - r->done);
-    NB_TRY(direct_step(s, fresh, true));
-    for (Request* r : fresh) r->have_acc = true;
-    NB_TRY(direct_step(s, direct, false, kdir));
+    int rc = direct_step(s, fresh, true);
+    if (rc == NB_OK) {
+        for (Request* r : fresh) r->have_acc = true;
+        rc = direct_step(s, direct, false, kdir);
+    }
+    for (Request* r : direct) {
+        if (rc == NB_OK) r->stepped = kdir;
+        else fail_request(r, rc);
+    }
     NB_HIP(hipDeviceSynchronize());
+    if (!bh.empty()) {
+        std::vector<int> flags(bh.size());
+        NB_HIP(hipMemcpy(flags.data(), s->d_flags.ptr, flags.size() * sizeof(int),
+                         hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < bh.size(); ++k) {
+            if (flags[k] == 0 || bh[k]->status != NB_OK) continue;
+            set_error("Barnes-Hut request %lld: invalid evaluation (flags %d: 1=walk frontier, "
+                      "2=node pool, 4=tree consistency, 8=sort look-back wait, 16=non-finite "
+                      "position or mass)", (long long)bh[k]->id, flags[k]);
+            fail_request(bh[k], (flags[k] & ~16) ? NB_ERR_CAPACITY : NB_ERR_INVALID);
+        }
+    }
     int nc = 0;
     std::vector<std::unique_ptr<Request>> still;
     for (auto& up : s->active) {
         Request* r = up.get();
-        if (r->done < r->desc.steps) {
-            const int k = r->bh ? 1 : kdir;
-            r->done += k;
-            s->particle_steps += r->desc.n * k;
-        }
-        if (r->done >= r->desc.steps) {
+        r->done += r->stepped;
+        s->particle_steps += r->desc.n * r->stepped;
+        if (r->done >= r->desc.steps || r->status != NB_OK) {
             for (int k = 1; k < 7; ++k) r->host[k].resize(r->desc.n);
-            NB_TRY(nb_particles_download(&r->p, r->host[1].data(), r->host[2].data(),
-                                         r->host[3].data(), r->host[4].data(), r->host[5].data(),
-                                         r->host[6].data(), nullptr, nullptr, nullptr, nullptr));
+            const int drc = nb_particles_download(&r->p, r->host[1].data(), r->host[2].data(),
+                                                  r->host[3].data(), r->host[4].data(),
+                                                  r->host[5].data(), r->host[6].data(), nullptr,
+                                                  nullptr, nullptr, nullptr);
+            if (drc != NB_OK && r->status == NB_OK) fail_request(r, drc);
             release(s, r);
             s->completed_since_poll.push_back(r->id);
             s->done[r->id] = std::move(up);
@@ -489,8 +540,12 @@ int nb_server_result(nb_server* s, int64_t id, double* x, double* y, double* z, 
     double* dst[6] = {x, y, z, vx, vy, vz};
     for (int k = 0; k < 6; ++k)
         if (dst[k]) std::memcpy(dst[k], r->host[k + 1].data(), sizeof(double) * r->desc.n);
+    const int status = r->status;
+    if (status != NB_OK)
+        set_error("nb_server_result: request %lld failed after %d of %d steps: %s", (long long)id,
+                  r->done, r->desc.steps, r->message.c_str());
     s->done.erase(it);
-    return NB_OK;
+    return status;
 }
 
 int nb_server_set_direct_steps(nb_server* s, int steps_per_tick) {
@@ -500,6 +555,16 @@ int nb_server_set_direct_steps(nb_server* s, int steps_per_tick) {
         return NB_ERR_INVALID;
     }
     s->direct_steps = steps_per_tick;
+    return NB_OK;
+}
+
+int nb_server_set_bh_steps(nb_server* s, int steps_per_tick) {
+    if (!s || steps_per_tick < 1 || steps_per_tick > kDirectStepsPerTickMax) {
+        set_error("nb_server_set_bh_steps: steps_per_tick=%d outside [1, %d]", steps_per_tick,
+                  kDirectStepsPerTickMax);
+        return NB_ERR_INVALID;
+    }
+    s->bh_steps = steps_per_tick;
     return NB_OK;
 }
 

@@ -54,7 +54,8 @@ class _BHStats(C.Structure):
                 ("pp_interactions", C.c_int64), ("pc_interactions", C.c_int64),
                 ("max_frontier", C.c_int32), ("frontier_overflow", C.c_int32),
                 ("root_center", C.c_double * 3), ("root_size", C.c_double),
-                ("n_groups", C.c_int64)]
+                ("n_groups", C.c_int64), ("flags_since_read", C.c_int32),
+                ("leaf_capacity", C.c_int32)]
 
 
 class _Diag(C.Structure):
@@ -109,6 +110,9 @@ def _load():
         "nb_force_bh_stats": ([_vp, C.POINTER(_BHStats)], C.c_int),
         "nb_force_bh_topology": ([_vp, C.c_int64, _vp, _vp, _vp, _vp], C.c_int),
         "nb_force_bh_nodes": ([_vp, C.c_int64, C.POINTER(C.c_int64)] + [_vp] * 7, C.c_int),
+        "nb_bh_set_tuning_file": ([C.c_char_p], C.c_int),
+        "nb_bh_auto_leaf_capacity": ([C.c_int64, C.POINTER(C.c_int)], C.c_int),
+        "nb_bh_node_capacity": ([C.c_int64, C.c_int], C.c_int64),
         "nb_integrator_create": ([C.POINTER(_vp), C.c_int], C.c_int),
         "nb_integrator_destroy": ([_vp], C.c_int),
         "nb_integrator_step": ([_vp, _vp, C.POINTER(_Particles), C.c_double], C.c_int),
@@ -122,6 +126,8 @@ def _load():
                            C.c_double] + [_D] * 7, C.c_int),
         "nb_ic_uniform_gaussian": ([C.c_int64, C.c_uint64, C.c_double] + [_D] * 7, C.c_int),
         "nb_ic_plummer_disk": ([C.c_int64, C.c_uint64] + [C.c_double] * 6 + [_D] * 7, C.c_int),
+        "nb_ic_reference_fixture": ([C.c_int64, C.c_uint32, C.c_int, C.c_int64] + [_D] * 4,
+                                    C.c_int),
         "nb_file_read": ([C.c_char_p, C.POINTER(C.c_int64)] + [_Dn] * 7 + [C.c_int64], C.c_int),
         "nb_file_write": ([C.c_char_p, C.c_int64] + [_D] * 7, C.c_int),
         "nb_ensemble_create": ([C.POINTER(_vp), C.c_int, _I64, _D, _D, _D], C.c_int),
@@ -142,6 +148,7 @@ def _load():
         "nb_server_result": ([_vp, C.c_int64] + [_Dn] * 6, C.c_int),
         "nb_server_get_status": ([_vp, C.POINTER(_ServerStatus)], C.c_int),
         "nb_server_set_direct_steps": ([_vp, C.c_int], C.c_int),
+        "nb_server_set_bh_steps": ([_vp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -320,7 +327,8 @@ class BarnesHutForceCalculator(ForceCalculator):
                     pp_interactions=s.pp_interactions, pc_interactions=s.pc_interactions,
                     max_frontier=s.max_frontier, frontier_overflow=s.frontier_overflow,
                     root_center=np.array(s.root_center[:]), root_size=s.root_size,
-                    n_groups=s.n_groups, status=rc)
+                    n_groups=s.n_groups, flags_since_read=s.flags_since_read,
+                    leaf_capacity=s.leaf_capacity, status=rc)
 
     def topology(self, n):
         depth = np.zeros(n, np.int32)
@@ -346,6 +354,26 @@ class BarnesHutForceCalculator(ForceCalculator):
                                      path.ctypes.data, first.ctypes.data, last.ctypes.data,
                                      mass.ctypes.data, com.ctypes.data, quad.ctypes.data))
         return dict(level=level, path=path, first=first, last=last, mass=mass, com=com, quad=quad)
+
+
+NB_LEAF_AUTO = 0
+
+
+def set_bh_tuning_file(path=None):
+    """Leaf-capacity table of NB_LEAF_AUTO calculators ("n_min cap" lines);
+    None returns to the default table."""
+    _check(lib.nb_bh_set_tuning_file(None if path is None else os.fsencode(path)))
+
+
+def auto_leaf_capacity(n) -> int:
+    c = C.c_int(0)
+    _check(lib.nb_bh_auto_leaf_capacity(int(n), C.byref(c)))
+    return c.value
+
+
+def bh_node_capacity(n, leaf_capacity) -> int:
+    """Node pool a Barnes-Hut calculator reserves (at most 2^28 nodes)."""
+    return int(lib.nb_bh_node_capacity(int(n), int(leaf_capacity)))
 
 
 class ForceCalculatorBarnesHutOptimized(BarnesHutForceCalculator):
@@ -485,6 +513,16 @@ def ic_plummer_disk(n, seed, a=1.0, M=1.0, G=1.0, disk_frac=0.1, rd=1.0, zd=0.05
     return dict(zip(("m", "x", "y", "z", "vx", "vy", "vz"), arr))
 
 
+def ic_reference_fixture(n, seed=42, order=0, skip=0):
+    """The reference's fixture stream (std::mt19937(seed), mass U[0.5,2),
+    positions U[-100,100)): order 0 = m,x,y,z (particles_*.txt), 1 = z,y,x,m,
+    2 = x,y,z,m; the first `skip` particles are drawn and dropped.  Zero
+    velocities."""
+    m, x, y, z = (np.zeros(n) for _ in range(4))
+    _check(lib.nb_ic_reference_fixture(int(n), int(seed), int(order), int(skip), m, x, y, z))
+    return dict(m=m, x=x, y=y, z=z, vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n))
+
+
 # --------------------------------------------------------------------------- particle files
 def read_particles(path) -> dict:
     n = C.c_int64(0)
@@ -559,7 +597,7 @@ class Server:
     """Continuous-batching server (see nb_server_* in include/nbody_amd.h)."""
 
     def __init__(self, hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=8,
-                 leaf_capacity=8, direct_steps_per_tick=None):
+                 leaf_capacity=8, direct_steps_per_tick=None, bh_steps_per_tick=None):
         h = C.c_void_p()
         _check(lib.nb_server_create(C.byref(h), float(hbm_budget_bytes), int(direct_threshold),
                                     int(bh_lanes), int(leaf_capacity)))
@@ -567,6 +605,8 @@ class Server:
         self._n = {}
         if direct_steps_per_tick is not None:
             _check(lib.nb_server_set_direct_steps(h, int(direct_steps_per_tick)))
+        if bh_steps_per_tick is not None:
+            _check(lib.nb_server_set_bh_steps(h, int(bh_steps_per_tick)))
 
     def submit(self, system, steps, G=1.0, softening=0.005, dt=1e-4, theta=0.5):
         arr = [_f64(system[k]) for k in ("m", "x", "y", "z", "vx", "vy", "vz")]
@@ -597,15 +637,21 @@ class Server:
 
     def result(self, rid):
         """Final state of a finished request (collected once).  An id that is
-        not finished, unknown or already collected raises NBError."""
+        not finished, unknown or already collected raises NBError; so does a
+        request that failed, with the state it reached as the error's `state`."""
         n = self._n.get(rid)
         if n is None:  # unknown here or already collected: the library says which
             _check(lib.nb_server_result(self._h, int(rid), *([None] * 6)))
             raise NBError(6, f"request {rid} was not submitted to this server")
         out = {k: np.empty(n) for k in ("x", "y", "z", "vx", "vy", "vz")}
-        _check(lib.nb_server_result(self._h, rid, *[out[k].ctypes.data for k in
-                                                    ("x", "y", "z", "vx", "vy", "vz")]))
-        del self._n[rid]
+        rc = lib.nb_server_result(self._h, rid, *[out[k].ctypes.data for k in
+                                                  ("x", "y", "z", "vx", "vy", "vz")])
+        if rc != 6:  # anything but "not finished" collects the request
+            del self._n[rid]
+        if rc != 0:
+            err = NBError(rc, lib.nb_last_error().decode())
+            err.state = out
+            raise err
         return out
 
     def status(self):

@@ -334,6 +334,83 @@ def test_frontier_overflow_is_reported(nb, monkeypatch):
     assert g.stats()["frontier_overflow"] == 0 and g.stats()["max_frontier"] > 16
 
 
+def test_frontier_overflow_nans_only_its_groups(nb, monkeypatch):
+    """After a frontier overflow the targets of the walk groups that
+    overflowed get NaN accelerations, never incomplete sums; every other
+    target's acceleration is bit-identical to an evaluation without overflow."""
+    pl = nb.ic_plummer(65536, 3)
+    _, full = run_bh(nb, pl, eps=0.005, cap=8, G=1.0)
+    monkeypatch.setenv("NB_WALK_FCAP", "48")
+    _, part = run_bh(nb, pl, eps=0.005, cap=8, G=1.0)
+    a, b = np.stack(full.acc(), 1), np.stack(part.acc(), 1)
+    bad = np.isnan(b).any(axis=1)
+    assert 0 < bad.sum() < len(bad)
+    assert np.isnan(b[bad]).all() and np.isfinite(a).all()
+    assert np.array_equal(a[~bad], b[~bad])
+
+
+def test_error_flags_persist_until_read(nb, monkeypatch):
+    """The error flags of every evaluation since the last stats() read are
+    kept (flags_since_read): an overflow in the first of two evaluations is
+    still reported after a clean second one, and reading clears it.  The same
+    holds inside a multi-step integrator run, where the overflowing step's
+    NaNs also reach the final state."""
+    pl = nb.ic_plummer(65536, 3)
+    f = nb.BarnesHutForceCalculator(0.005, 0.5, 8, "quadrupole", G=1.0)
+    p = nb.Particles.from_arrays(pl["m"], pl["x"], pl["y"], pl["z"])
+    monkeypatch.setenv("NB_WALK_FCAP", "16")
+    f.computeAccelerations(p)
+    monkeypatch.delenv("NB_WALK_FCAP")
+    f.computeAccelerations(p)
+    st = f.stats(check=False)
+    assert st["frontier_overflow"] == 0 and st["flags_since_read"] & 1
+    assert st["status"] == 4  # NB_ERR_CAPACITY
+    assert f.stats()["flags_since_read"] == 0  # the read cleared it
+
+    q = nb.Particles.from_arrays(pl["m"], pl["x"], pl["y"], pl["z"], pl["vx"], pl["vy"], pl["vz"])
+    g = nb.BarnesHutForceCalculator(0.005, 0.5, 8, "quadrupole", G=1.0)
+    it = nb.LeapfrogIntegrator()
+    monkeypatch.setenv("NB_WALK_FCAP", "16")
+    # The step's first evaluation overflows; its NaNs move the affected
+    # particles to NaN positions, so the step's second evaluation sees a
+    # non-finite input (flag 16).  Both are reported.
+    it.step(g, q, 1e-4)
+    monkeypatch.delenv("NB_WALK_FCAP")
+    st = g.stats(check=False)
+    assert st["flags_since_read"] & 1 and st["flags_since_read"] & 16
+    assert st["status"] == 4  # NB_ERR_CAPACITY: the overflow is the cause
+    assert np.isnan(q.download()["x"]).any()
+
+
+def test_auto_leaf_capacity_follows_the_table(nb, O, tmp_path):
+    """leaf_capacity = NB_LEAF_AUTO takes the capacity of the table row for n
+    (here a file in the leaf_capacity_sweep format): the tree equals an
+    explicit calculator's at that capacity bit for bit, and the forces meet
+    the Barnes-Hut accuracy bar."""
+    t = tmp_path / "caps.txt"
+    t.write_text("0 4\n20000 16\n")
+    nb.set_bh_tuning_file(t)
+    try:
+        for n, cap in ((5000, 4), (40000, 16)):
+            d = nb.ic_plummer(n, 31)
+            auto, pa = run_bh(nb, d, eps=0.005, cap=nb.NB_LEAF_AUTO, G=1.0)
+            fixed, pf = run_bh(nb, d, eps=0.005, cap=cap, G=1.0)
+            assert auto.stats()["leaf_capacity"] == cap
+            ta, tf = auto.topology(n), fixed.topology(n)
+            assert all(np.array_equal(ta[k], tf[k]) for k in ta)
+            assert all(np.array_equal(a, b) for a, b in zip(pa.acc(), pf.acc()))
+            r = O.direct_subset(d["m"], d["x"], d["y"], d["z"], np.arange(0, n, 7), G=1.0,
+                                soft2=0.005 ** 2)
+            a = np.stack(pa.acc(), 1)[::7]
+            rel = np.linalg.norm(a - np.stack(r, 1), axis=1) / np.linalg.norm(np.stack(r, 1), axis=1)
+            assert np.percentile(rel, 99) < 1e-3
+    finally:
+        nb.set_bh_tuning_file(None)
+    d = nb.ic_plummer(3000, 5)
+    auto, _ = run_bh(nb, d, eps=0.005, cap=nb.NB_LEAF_AUTO, G=1.0)
+    assert auto.stats()["leaf_capacity"] == nb.auto_leaf_capacity(3000)
+
+
 def test_node_pool_overflow_is_reported(nb, fixtures, monkeypatch):
     """A node pool smaller than the tree (NB_BH_NODE_CAP, a testing override)
     sets error bit 2: nothing is written past the pool, the accelerations are

@@ -180,3 +180,20 @@ def test_cpp_drop_in_example(O, golden, fixtures, tmp_path):
     assert "(recursive) build_ms=" in r.stdout and "calls=1" in r.stdout.split("(recursive)")[1]
     e = rel_err(tuple(rec), ref)
     assert np.percentile(e, 99) < 1e-2
+
+
+def test_zero_step_runs_leave_the_state_unchanged(nb):
+    """Zero steps, batched, graph or one system at a time, change nothing:
+    no closing kick is applied without a step."""
+    sizes = [100, 257, 64]
+    e = nb.Ensemble(sizes, G=1.0, softening=0.01, dt=1e-3)
+    parts = [nb.ic_plummer(n, 40 + k) for k, n in enumerate(sizes)]
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in ("m", "x", "y", "z", "vx", "vy", "vz")}
+    e.upload(cat["m"], cat["x"], cat["y"], cat["z"], cat["vx"], cat["vy"], cat["vz"])
+    before = e.download()
+    e.run_unbatched(0)
+    e.run(0, use_graph=False)
+    e.run(0, use_graph=True)
+    after = e.download()
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(before[k], after[k]), k

@@ -5,6 +5,8 @@ in a mix (direct requests share one batched launch, Barnes-Hut requests share
 lanes); a Barnes-Hut request equals the library's own BH + leapfrog path bit
 for bit; direct requests agree with the FP64 oracle to FP32 tolerance;
 admission respects the HBM budget."""
+import re
+
 import numpy as np
 import pytest
 
@@ -20,9 +22,9 @@ def systems(nb):
     return [nb.ic_plummer(n, seed) for n, _, seed in REQS]
 
 
-def run_server(nb, items, stagger=0, budget=200e9, lanes=2, kdir=None):
+def run_server(nb, items, stagger=0, budget=200e9, lanes=2, kdir=None, kbh=None):
     srv = nb.Server(hbm_budget_bytes=budget, direct_threshold=32768, bh_lanes=lanes,
-                    direct_steps_per_tick=kdir)
+                    direct_steps_per_tick=kdir, bh_steps_per_tick=kbh)
     ids = []
     pending = list(items)
     results = {}
@@ -206,3 +208,55 @@ def test_bh_lane_workspace_accounting(nb):
         nb.Server(hbm_budget_bytes=state * 2, direct_threshold=1000, bh_lanes=2).submit(sysd, 1)
     assert e.value.code == 5
 
+
+
+def test_bh_steps_per_tick_does_not_change_results(nb):
+    """Barnes-Hut requests take up to B steps per tick (fused leapfrog runs);
+    every B gives the same final states, bit for bit, and the particle-step
+    count is exact."""
+    sy = systems(nb)
+    items = [(sy[4], 7), (sy[0], 5), (sy[5], 4), (sy[1], 8)]
+    base, _ = run_server(nb, items, stagger=2, kbh=1)
+    for k in (3, 8):
+        res, srv = run_server(nb, items, stagger=2, kbh=k)
+        for a, b in zip(base, res):
+            assert all(np.array_equal(a[f], b[f]) for f in FIELDS)
+        assert srv.status()["particle_steps"] == sum(len(sd["m"]) * st for sd, st in items)
+    for bad in (0, 65):
+        with pytest.raises(nb.NBError):
+            nb.Server(bh_steps_per_tick=bad)
+
+
+def test_failed_bh_request_fails_alone(nb, monkeypatch):
+    """A Barnes-Hut request whose walk frontier overflows (NB_WALK_FCAP, a
+    testing override that keeps every kernel in bounds) finishes at once with
+    NB_ERR_CAPACITY, its state attached to the error; the direct requests of
+    the same ticks finish normally, bit-identical to running alone, and its
+    memory is released."""
+    sy = systems(nb)
+    direct_items = [(sy[0], 5), (sy[1], 8)]
+    alone = [run_server(nb, [it])[0][0] for it in direct_items]
+    monkeypatch.setenv("NB_WALK_FCAP", "4")
+    srv = nb.Server(hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=2)
+    bad = srv.submit(sy[5], 50)
+    ids = [srv.submit(s, k) for s, k in direct_items]
+    done, ticks = {}, 0
+    while len(done) < 3:
+        srv.tick()
+        ticks += 1
+        for rid in srv.poll():
+            try:
+                done[rid] = srv.result(rid)
+            except nb.NBError as e:
+                assert rid == bad and e.code == 4  # NB_ERR_CAPACITY
+                # flag 1 (frontier), and 16 when the NaNs reached the step's
+                # second evaluation as non-finite positions
+                assert re.search(r"flags (1|17):", str(e)), str(e)
+                assert e.state["x"].shape == (len(sy[5]["m"]),)
+                done[rid] = None
+                assert ticks == 1  # failed in the tick of its first step
+        assert ticks < 100
+    for rid, a in zip(ids, alone):
+        assert all(np.array_equal(done[rid][f], a[f]) for f in FIELDS)
+    st = srv.status()
+    assert st["active"] == 0 and st["pending"] == 0
