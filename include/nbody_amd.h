@@ -141,6 +141,19 @@ int nb_force_destroy(nb_force* f);
  * also shows in every later state that depends on it. */
 int nb_force_compute(nb_force* f, nb_particles* p);
 
+/* No counterpart in the reference: the NB_DIRECT_F32 force of
+ * nb_force_compute for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It allocates nothing: the packed FP32 copy of the sources lives in a
+ * library-owned static device array of NB_NOALLOC_MAX_N float4 (16 MiB per
+ * GPU, held until the process ends), so n > NB_NOALLOC_MAX_N is rejected with
+ * NB_ERR_CAPACITY.  Writes p->ax, p->ay, p->az with the same bits as an
+ * NB_DIRECT_F32 calculator.  Asynchronous on `stream` (a hipStream_t; NULL =
+ * the null stream).  All calls on one GPU share its array: calls that
+ * overlap on one GPU (different streams or host threads) race. */
+#define NB_NOALLOC_MAX_N (1 << 20)
+int nb_force_direct_f32_noalloc(nb_particles* p, double G, double softening, void* stream);
+
 /* ForceCalculator::name() (CPU/include/force_calculator.hpp:28). */
 const char* nb_force_name(const nb_force* f);
 

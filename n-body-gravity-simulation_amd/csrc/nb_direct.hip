@@ -13,7 +13,9 @@
 //  * k_direct_f32: FP32 with v_rsq_f32, 3 sub + 3 fma + rsq + 3 mul + 3 fma
 //    per interaction, R targets per lane to amortise the LDS broadcast.
 #include <climits>
+#include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 #include "nb_internal.hpp"
 
@@ -186,6 +188,32 @@ static void launch_f32(int n, const float4* pm, float soft2, double* ax, double*
                            soft2, ax, ay, az);
 }
 
+// Packs (x, y, z, G*m) to FP32 into `pm` and launches the FP32 tiled kernel.
+static int direct_f32(int n, const double* m, const double* x, const double* y, const double* z,
+                      double G, double soft2, float4* pm, int rows, double* ax, double* ay,
+                      double* az, hipStream_t s) {
+    const int grid = grid_for(n, kDirectBlock);
+    hipLaunchKernelGGL(k_pack_f32, dim3(grid), dim3(kDirectBlock), 0, s, n, x, y, z, m, G, pm);
+    const float s2 = (float)soft2;
+    if (rows >= 4)
+        launch_f32<4>(n, pm, s2, ax, ay, az, s);
+    else if (rows == 2)
+        launch_f32<2>(n, pm, s2, ax, ay, az, s);
+    else
+        launch_f32<1>(n, pm, s2, ax, ay, az, s);
+    NB_LAUNCH_CHECK();
+    return NB_OK;
+}
+
+// Targets per lane of the FP32 kernel: NB_DIRECT_ROWS overrides the default 1.
+// Every value gives the same bits (each target sums its sources in the same
+// order); it only changes how the work is laid out.
+static int direct_rows() {
+    int rows = 1;
+    if (const char* e = std::getenv("NB_DIRECT_ROWS")) rows = std::atoi(e);
+    return rows;
+}
+
 int direct_eval(int precision, double G, double soft2, int64_t n64, const double* m,
                 const double* x, const double* y, const double* z, double* ax, double* ay,
                 double* az, DirectWS& ws, hipStream_t s) {
@@ -210,22 +238,10 @@ int direct_eval(int precision, double G, double soft2, int64_t n64, const double
                                    n, x, y, z, m, G, soft2, ax, ay, az);
             break;
         case NB_DIRECT_F32: {
-            if (ws.rows == 0) {  // targets per lane: NB_DIRECT_ROWS overrides the default
-                ws.rows = 1;
-                if (const char* e = std::getenv("NB_DIRECT_ROWS")) ws.rows = std::atoi(e);
-            }
+            if (ws.rows == 0) ws.rows = direct_rows();
             NB_TRY(ws.pack.reserve((size_t)n * sizeof(float4)));
-            float4* pm = ws.pack.as<float4>();
-            hipLaunchKernelGGL(k_pack_f32, dim3(grid), dim3(kDirectBlock), 0, s, n, x, y, z, m, G,
-                               pm);
-            const float s2 = (float)soft2;
-            if (ws.rows >= 4)
-                launch_f32<4>(n, pm, s2, ax, ay, az, s);
-            else if (ws.rows == 2)
-                launch_f32<2>(n, pm, s2, ax, ay, az, s);
-            else
-                launch_f32<1>(n, pm, s2, ax, ay, az, s);
-            break;
+            return direct_f32(n, m, x, y, z, G, soft2, ws.pack.as<float4>(), ws.rows, ax, ay, az,
+                              s);
         }
         default:
             set_error("direct force: unknown precision %d", precision);
@@ -235,4 +251,68 @@ int direct_eval(int precision, double G, double soft2, int64_t n64, const double
     return NB_OK;
 }
 
+// ------------------------------------------------- allocation-free variant
+// The packed FP32 sources of nb_force_direct_f32_noalloc live in this
+// file-scope device array (one copy per GPU, held until the process ends)
+// instead of a calculator workspace, so the call allocates nothing.  Host
+// code never uses the array's name: its device address is looked up once per
+// device with hipGetSymbolAddress and kept in g_scratch.
+constexpr int64_t kNoAllocMaxN = NB_NOALLOC_MAX_N;  // 16 MiB of float4
+static_assert(kNoAllocMaxN == ((int64_t)1 << 20), "scratch size changed: update the header");
+static __device__ __align__(256) float4 scratch[kNoAllocMaxN];
+
+constexpr int kNoAllocMaxDevices = 64;
+
+namespace {
+std::mutex g_scratch_mu;
+float4* g_scratch[kNoAllocMaxDevices] = {};  // device address of `scratch`, per device
+}  // namespace
+
+static int scratch_for_current_device(float4** out) {
+    int dev = 0;
+    NB_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) {
+        set_error("direct force (no-alloc): device %d beyond the %d supported", dev,
+                  kNoAllocMaxDevices);
+        return NB_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        NB_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) {
+            set_error("direct force (no-alloc): no device address for the scratch array");
+            return NB_ERR_HIP;
+        }
+        g_scratch[dev] = static_cast<float4*>(ptr);
+    }
+    *out = g_scratch[dev];
+    return NB_OK;
+}
+
 }  // namespace nb
+
+using namespace nb;
+
+extern "C" int nb_force_direct_f32_noalloc(nb_particles* p, double G, double softening,
+                                           void* stream) {
+    if (!p || !std::isfinite(G) || !std::isfinite(softening) || p->n < 0) {
+        set_error("nb_force_direct_f32_noalloc: invalid arguments");
+        return NB_ERR_INVALID;
+    }
+    if (p->n > kNoAllocMaxN) {
+        set_error("nb_force_direct_f32_noalloc: n=%lld exceeds the scratch capacity of %lld "
+                  "particles (use nb_force_compute with an NB_DIRECT_F32 calculator)",
+                  (long long)p->n, (long long)kNoAllocMaxN);
+        return NB_ERR_CAPACITY;
+    }
+    if (p->n == 0) return NB_OK;
+    if (!p->x || !p->y || !p->z || !p->m || !p->ax || !p->ay || !p->az) {
+        set_error("nb_force_direct_f32_noalloc: null particle array");
+        return NB_ERR_INVALID;
+    }
+    float4* pm = nullptr;
+    NB_TRY(scratch_for_current_device(&pm));
+    return direct_f32((int)p->n, p->m, p->x, p->y, p->z, G, softening * softening, pm,
+                      direct_rows(), p->ax, p->ay, p->az, static_cast<hipStream_t>(stream));
+}

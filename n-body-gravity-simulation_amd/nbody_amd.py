@@ -103,6 +103,8 @@ def _load():
         "nb_force_destroy": ([_vp], C.c_int),
         "nb_force_compute": ([_vp, C.POINTER(_Particles)], C.c_int),
         "nb_force_name": ([_vp], C.c_char_p),
+        "nb_force_direct_f32_noalloc": ([C.POINTER(_Particles), C.c_double, C.c_double, _vp],
+                                        C.c_int),
         "nb_force_set_timing": ([_vp, C.c_int], C.c_int),
         "nb_force_timing": ([_vp, _D], C.c_int),
         "nb_force_stage_timing": ([_vp, _D, C.c_int], C.c_int),
@@ -357,6 +359,16 @@ class BarnesHutForceCalculator(ForceCalculator):
 
 
 NB_LEAF_AUTO = 0
+NB_NOALLOC_MAX_N = 1 << 20
+
+
+def direct_f32_noalloc(particles: Particles, softening: float = 0.0, G: float = G_REF,
+                       stream=None):
+    """The NB_DIRECT_F32 force without any device allocation (the packed
+    sources live in the library's static scratch array; n <= NB_NOALLOC_MAX_N).
+    Asynchronous on `stream` (a hipStream_t handle as an int, None = null
+    stream)."""
+    _check(lib.nb_force_direct_f32_noalloc(particles.ref, float(G), float(softening), stream))
 
 
 def set_bh_tuning_file(path=None):
