@@ -7,6 +7,8 @@
 #ifndef NBODY_AMD_DIAG_H
 #define NBODY_AMD_DIAG_H
 
+#include "nbody_amd.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -23,6 +25,13 @@ int nb_probe_fp32_fma(double* tflops, double* ms);
  * rsqrt therefore issues at most at k + 1 instructions per (k + *fma_slots)
  * FMA slots.  Synchronous. */
 int nb_probe_fp32_rsq_cost(double* fma_slots);
+
+/* Mean duration in ms of an ensemble's batched force launch (one persistent
+ * k_ens_force over all systems), from `reps` back-to-back launches between HIP
+ * events on the ensemble's stream.  The launches recompute the cached
+ * acceleration from the same positions, so the state is unchanged.
+ * Synchronous. */
+int nb_ensemble_time_force(nb_ensemble* e, int reps, double* ms);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,7 @@
 #include <numeric>
 #include <sstream>
 
+#include "../../include/nbody_amd_diag.h"
 #include "nb_internal.hpp"
 
 namespace nb {
@@ -455,6 +456,36 @@ int nb_ensemble_load_tuning(nb_ensemble* e, const char* path) {
     NB_HIP(hipStreamSynchronize(e->stream));
     std::copy(sh, sh + kEnsBuckets, e->shape);
     return ens_build_items(e);
+}
+
+// Mean duration of the batched force launch (k_ens_force over every system)
+// from HIP events on the ensemble's stream.  The launches recompute the
+// cached acceleration from the same positions with the same shapes, so the
+// state keeps its bits.
+int nb_ensemble_time_force(nb_ensemble* e, int reps, double* ms) {
+    if (!e || reps < 1 || !ms) {
+        set_error("nb_ensemble_time_force: invalid arguments");
+        return NB_ERR_INVALID;
+    }
+    NB_TRY(ens_prime(e));
+    hipEvent_t a, b;
+    NB_HIP(hipEventCreate(&a));
+    NB_HIP(hipEventCreate(&b));
+    int st = NB_OK;
+    float t = 0.f;
+    if (hipEventRecord(a, e->stream) != hipSuccess) st = NB_ERR_HIP;
+    for (int r = 0; r < reps && st == NB_OK; ++r)
+        st = ens_force(e, e->d_items.as<EnsItem>(), e->n_items, e->stream);
+    if (st == NB_OK && (hipEventRecord(b, e->stream) != hipSuccess ||
+                        hipEventSynchronize(b) != hipSuccess ||
+                        hipEventElapsedTime(&t, a, b) != hipSuccess)) {
+        set_error("nb_ensemble_time_force: event timing failed");
+        st = NB_ERR_HIP;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = (double)t / reps;
+    return st;
 }
 
 // Times the three shapes on each occupied size bucket of this ensemble (one

@@ -141,6 +141,7 @@ def _load():
         "nb_ensemble_autotune": ([_vp, C.c_char_p], C.c_int),
         "nb_ensemble_load_tuning": ([_vp, C.c_char_p], C.c_int),
         "nb_ensemble_tuning": ([_vp, _vp, C.c_int], C.c_int),
+        "nb_ensemble_time_force": ([_vp, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "nb_server_create": ([C.POINTER(_vp), C.c_double, C.c_int64, C.c_int, C.c_int], C.c_int),
         "nb_server_destroy": ([_vp], C.c_int),
         "nb_server_submit": ([_vp, C.POINTER(_Request), C.POINTER(C.c_int64)], C.c_int),
@@ -589,6 +590,13 @@ class Ensemble:
 
     def load_tuning(self, path):
         _check(lib.nb_ensemble_load_tuning(self._h, os.fsencode(path)))
+
+    def time_force(self, reps=5):
+        """Mean ms of one batched force launch (HIP events on the ensemble's
+        stream); the state keeps its bits."""
+        ms = C.c_double(0.0)
+        _check(lib.nb_ensemble_time_force(self._h, int(reps), C.byref(ms)))
+        return ms.value
 
     def tuning(self):
         out = np.zeros(8, np.int32)

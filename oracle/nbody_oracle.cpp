@@ -18,6 +18,7 @@
 //                                 insertion tree (leaf path == key prefix)
 //   diag                          long-double sums vs closed forms / brute force
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -588,6 +589,32 @@ void orc_bhopt(int64_t n, double G, double soft2, double theta, int leaf_cap, co
         stats[2] = maxd;
         stats[3] = max_stack;
     }
+}
+
+// Times orc_bhopt on a bounded sample (CPU baseline, not a checker): the full
+// tree build, then the walk of n_sample targets spaced n / n_sample apart
+// (OpenMP over targets, as the reference walks them).  ms = {build, walk}.
+void orc_bhopt_sample(int64_t n, double G, double soft2, double theta, int leaf_cap,
+                      const double* m, const double* x, const double* y, const double* z,
+                      int64_t n_sample, double* ax, double* ay, double* az, double* ms) {
+    SoA s{n, m, x, y, z};
+    PoolTree t;
+    auto t0 = std::chrono::steady_clock::now();
+    t.build(s, leaf_cap);
+    auto t1 = std::chrono::steady_clock::now();
+    const double theta_sqr = theta * theta;
+    const int64_t stride = n_sample > 0 && n / n_sample > 0 ? n / n_sample : 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n_sample; ++k) {
+        int hi = 0;
+        const V3 a = t.walk((k * stride) % n, G, soft2, theta_sqr, &hi);
+        ax[k] = a.x;
+        ay[k] = a.y;
+        az[k] = a.z;
+    }
+    auto t2 = std::chrono::steady_clock::now();
+    ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ms[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
 }
 
 // Canonical topology of the optimized Barnes-Hut tree: for every particle the

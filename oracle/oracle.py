@@ -45,6 +45,7 @@ def lib():
         L.orc_rk4_direct.argtypes = [_i64, _d, _d, _d, _i, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
         L.orc_step1_direct.argtypes = [_i64, _i, _d, _d, _d, _i] + [_D] * 10
         L.orc_bhopt.argtypes = [_i64, _d, _d, _d, _i, _D, _D, _D, _D, _D, _D, _D, C.c_void_p]
+        L.orc_bhopt_sample.argtypes = [_i64, _d, _d, _d, _i, _D, _D, _D, _D, _i64, _D, _D, _D, _D]
         L.orc_bhopt_leaves.argtypes = [_i64, _i, _D, _D, _D, _D, _I32, _U64, C.POINTER(C.c_int64)]
         L.orc_bhopt_leaves.restype = C.c_int64
         L.orc_bhrec.argtypes = [_i64, _d, _d, _d, _D, _D, _D, _D, _D, _D, _D]
@@ -143,6 +144,16 @@ def bhopt(m, x, y, z, theta=0.5, leaf_cap=8, G=G_REF, soft2=0.0, with_stats=Fals
         return (ax, ay, az), dict(pool_nodes=int(stats[0]), internal=int(stats[1]),
                                   max_depth=int(stats[2]), max_stack=int(stats[3]))
     return ax, ay, az
+
+
+def bhopt_sample(m, x, y, z, n_sample, theta=0.5, leaf_cap=8, G=G_REF, soft2=0.0):
+    """Timing of bhopt on a sample (CPU baseline): full build, walk of
+    n_sample evenly spaced targets.  Returns (accelerations, {build_ms, walk_ms})."""
+    m, x, y, z = map(_c, (m, x, y, z))
+    ax, ay, az = (np.zeros(n_sample) for _ in range(3))
+    ms = np.zeros(2)
+    lib().orc_bhopt_sample(len(m), G, soft2, theta, leaf_cap, m, x, y, z, n_sample, ax, ay, az, ms)
+    return (ax, ay, az), {"build_ms": ms[0], "walk_ms": ms[1]}
 
 
 def bhopt_leaves(m, x, y, z, leaf_cap=8):

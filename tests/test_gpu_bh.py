@@ -483,8 +483,17 @@ def test_plummer_1m_full_size(nb, O):
     assert np.array_equal(np.sort(perm), np.arange(n))
     dep, path, _, _ = O.bhopt_leaves(pl["m"], pl["x"], pl["y"], pl["z"], 8)
     assert np.array_equal(top["depth"], dep) and np.array_equal(top["path"], path)
-    e = rel_err(p.acc(), f64_direct(nb, pl, 0.005, 1.0))
+    ref = f64_direct(nb, pl, 0.005, 1.0)
+    e = rel_err(p.acc(), ref)
     assert np.percentile(e, 99) <= 1e-3 and np.median(e) <= 3e-4
+    # The FP64 GPU reference itself, pinned on 2048 sampled targets to the
+    # oracle's FP64 direct sum (the reference CPU arithmetic).
+    idx = np.random.default_rng(7).choice(n, 2048, replace=False)
+    o = np.stack(O.direct_subset(pl["m"], pl["x"], pl["y"], pl["z"], idx, G=1.0,
+                                 soft2=0.005 ** 2), 1)
+    assert rel_err([r[idx] for r in ref], list(o.T)).max() < 1e-12
+    e_o = rel_err([a[idx] for a in p.acc()], list(o.T))
+    assert np.percentile(e_o, 99) <= 1e-3 and np.median(e_o) <= 3e-4
 
 
 def test_leaf_capacity_sweep_harness(O):

@@ -197,3 +197,84 @@ def test_zero_step_runs_leave_the_state_unchanged(nb):
     after = e.download()
     for k in ("x", "y", "z", "vx", "vy", "vz"):
         assert np.array_equal(before[k], after[k]), k
+
+
+def c4_sizes():
+    """The 4096-system ensemble workload: N log-uniform in [64, 8192] and
+    scale radius uniform in [0.5, 2] from one seeded stream."""
+    rng = np.random.Generator(np.random.Philox(4))
+    sizes = np.rint(np.exp(rng.uniform(np.log(64), np.log(8192), 4096))).astype(np.int64)
+    radii = rng.uniform(0.5, 2.0, 4096)
+    return sizes, radii
+
+
+def bucket(n):
+    return min(7, max(0, int(np.floor(np.log2(n))) - 6))
+
+
+def test_full_config_4096_systems_vs_oracle(nb, O, tmp_path):
+    """The full 4096-system workload, autotuned, 10 steps through the HIP
+    graph.  A seeded sample of 40 systems covering every occupied size bucket
+    (the largest systems of the [4096, 8192] range included) agrees with the
+    FP64 oracle (positions within 1e-8 after 10 steps, forces p99 < 1e-4); the
+    four largest systems give the same bits alone as in the batch."""
+    sizes, radii = c4_sizes()
+    parts = [nb.ic_plummer(int(n), 1000 + k, a=float(a)) for k, (n, a) in enumerate(zip(sizes, radii))]
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    e = new_ensemble(nb, sizes, cat)
+    tuning = tmp_path / "tuning.txt"
+    e.autotune(tuning)
+    e.run(10, use_graph=True)
+    g = e.download()
+    occupied = sorted({bucket(n) for n in sizes})
+    assert occupied[-1] >= 6
+    rng = np.random.default_rng(5)
+    by_size = np.argsort(sizes)
+    pick = list(by_size[-4:])  # the largest four
+    for b in occupied:
+        members = np.flatnonzero([bucket(n) == b for n in sizes])
+        pick += list(rng.choice(members, min(len(members), 5), replace=False))
+    pick = sorted(set(int(k) for k in pick))
+    assert len(pick) >= 32
+    for k in pick:
+        p, sl = parts[k], slice(off[k], off[k + 1])
+        st = O.leapfrog_direct(p["m"], p["x"], p["y"], p["z"], p["vx"], p["vy"], p["vz"], 1e-3, 10,
+                               1.0, 1e-4, False)
+        for f, ref in zip(("x", "y", "z"), st[:3]):
+            assert np.abs(g[f][sl] - ref).max() < 1e-8, (k, f)
+        e_rel = rel_err((g["ax"][sl], g["ay"][sl], g["az"][sl]), st[6:9])
+        assert np.percentile(e_rel, 99) < 1e-4, k
+    for k in by_size[-4:]:
+        p, sl = parts[k], slice(off[k], off[k + 1])
+        one = new_ensemble(nb, [sizes[k]], p)
+        one.load_tuning(tuning)
+        one.run(10, use_graph=True)
+        h = one.download()
+        for f in STATE:
+            assert np.array_equal(h[f], g[f][sl]), (k, f)
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_each_shape_code_vs_oracle(nb, O, tmp_path, shape):
+    """Every tile shape (64 targets x 4 source slices, 128 x 2, 256 x 1),
+    forced for all size buckets through load_tuning, agrees with the FP64
+    oracle on systems from one tile to the largest size."""
+    sizes = [64, 100, 300, 1500, 5000, 8192]
+    parts, cat = make_systems(nb, sizes, seed0=700)
+    t = tmp_path / "t.txt"
+    t.write_text("".join(f"{b} {shape}\n" for b in range(8)))
+    e = new_ensemble(nb, sizes, cat)
+    e.load_tuning(t)
+    assert list(e.tuning()) == [shape] * 8
+    e.run(5, use_graph=True)
+    g = e.download()
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for k, p in enumerate(parts):
+        sl = slice(off[k], off[k + 1])
+        st = O.leapfrog_direct(p["m"], p["x"], p["y"], p["z"], p["vx"], p["vy"], p["vz"], 1e-3, 5,
+                               1.0, 1e-4, False)
+        for f, ref in zip(("x", "y", "z"), st[:3]):
+            assert np.abs(g[f][sl] - ref).max() < 1e-8, (shape, k, f)
+        e_rel = rel_err((g["ax"][sl], g["ay"][sl], g["az"][sl]), st[6:9])
+        assert np.percentile(e_rel, 99) < 1e-4, (shape, k)

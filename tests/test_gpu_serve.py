@@ -260,3 +260,21 @@ def test_failed_bh_request_fails_alone(nb, monkeypatch):
         assert all(np.array_equal(done[rid][f], a[f]) for f in FIELDS)
     st = srv.status()
     assert st["active"] == 0 and st["pending"] == 0
+
+
+def test_million_particle_bh_request_equals_library_path(nb):
+    """A Barnes-Hut request at the top of the serving range (N = 10^6, the
+    size that sets p99), 3 steps through the server while a direct request
+    shares its ticks, is bit-identical to the library's own Barnes-Hut +
+    leapfrog path."""
+    n, steps = 1_000_000, 3
+    s = nb.ic_plummer(n, 77)
+    small = nb.ic_plummer(2000, 78)
+    res, srv = run_server(nb, [(s, steps), (small, 5)], lanes=2, kbh=2)
+    f = nb.BarnesHutForceCalculator(0.005, 0.5, 8, "quadrupole", G=1.0)
+    p = nb.Particles.from_arrays(s["m"], s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"])
+    nb.LeapfrogIntegrator().run(f, p, 1e-4, steps)
+    g = p.download()
+    for fl in FIELDS:
+        assert np.array_equal(g[fl], res[0][fl]), fl
+    assert srv.status()["particle_steps"] == n * steps + 2000 * 5
