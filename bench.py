@@ -625,7 +625,7 @@ def bench_serve(args):
     sustains for this request mix."""
     import nbody_amd as nb
 
-    rates = [10, 25, 50, 100, 250, 500]
+    rates = [int(r) for r in args.rates.split(",")]
     R = args.requests
     rng = np.random.Generator(np.random.Philox(1000))
     ns = np.rint(np.exp(rng.uniform(np.log(1e3), np.log(1e6), R))).astype(int)
@@ -646,7 +646,8 @@ def bench_serve(args):
         gaps = np.random.Generator(np.random.Philox(2000 + ri)).exponential(1.0 / rate, R)
         arrivals = np.cumsum(gaps) - gaps[0]
         srv = nb.Server(hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=args.bh_lanes,
-                        bh_steps_per_tick=args.bh_steps)
+                        bh_steps_per_tick=args.bh_steps, tick_budget=args.tick_budget,
+                        direct_steps_per_tick=args.direct_steps)
         t0 = time.perf_counter()
         sub = 0
         done, admitted, rid_of, failed = {}, {}, {}, 0
@@ -702,7 +703,8 @@ def bench_serve(args):
         res = []
         for _ in range(2):  # alone, then with the whole trace submitted at once
             srv = nb.Server(hbm_budget_bytes=200e9, direct_threshold=32768,
-                            bh_lanes=args.bh_lanes, bh_steps_per_tick=args.bh_steps)
+                            bh_lanes=args.bh_lanes, bh_steps_per_tick=args.bh_steps,
+                            tick_budget=args.tick_budget, direct_steps_per_tick=args.direct_steps)
             rid = srv.submit(systems[k], int(steps[k]), softening=0.005, dt=1e-4)
             if res:  # the mix: a few other requests sharing the ticks
                 for q in np.argsort(ns * steps)[:8]:
@@ -723,7 +725,11 @@ def bench_serve(args):
         "workload": (f"serving: one trace of {R} requests replayed with Poisson arrivals at "
                      f"{rates} req/s; N log-uniform [1e3,1e6], steps U[10,1000], eps=0.005, "
                      f"dt=1e-4, direct below 32768 else BH theta=0.5, HBM budget 200 GB, "
-                     f"{args.bh_lanes} BH lanes, {args.bh_steps} BH step(s) per tick"),
+                     f"{args.bh_lanes} BH lanes, "
+                     + (f"shortest remaining work first, {args.tick_budget:.3g} BH particle-steps "
+                        f"per tick" if args.tick_budget > 0 else
+                        f"{args.bh_steps} BH step(s) per request per tick")
+                     + f", {args.direct_steps} direct steps per tick"),
         "p50_s": {str(r["rate_per_s"]): round(r["p50_s"], 3) for r in rows},
         "p99_s": {str(r["rate_per_s"]): round(r["p99_s"], 3) for r in rows},
         "knee_rate_per_s": best / mean_work,
@@ -785,7 +791,13 @@ def main():
     ap.add_argument("--requests", type=int, default=150, help="serve: requests in the trace")
     ap.add_argument("--bh-lanes", type=int, default=8,
                     help="serve: Barnes-Hut calculators (streams) stepping concurrently")
-    ap.add_argument("--bh-steps", type=int, default=1, help="serve: Barnes-Hut steps per tick")
+    ap.add_argument("--bh-steps", type=int, default=1,
+                    help="serve: Barnes-Hut steps per request per tick (without a tick budget)")
+    ap.add_argument("--tick-budget", type=float, default=4e7,
+                    help="serve: Barnes-Hut particle-steps per tick, shortest remaining work first "
+                         "(0: every request every tick)")
+    ap.add_argument("--direct-steps", type=int, default=64, help="serve: direct steps per tick")
+    ap.add_argument("--rates", default="10,25,50,100,250,500", help="serve: arrival rates, req/s")
     ap.add_argument("--scale-n", type=int, default=1 << 24,
                     help="also report the tree stages at this N (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",

@@ -215,8 +215,11 @@ int nb_force_bh_topology(nb_force* f, int64_t n, int32_t* leaf_depth, uint64_t* 
 /* Multipole moments of every non-empty node of the last tree (synchronizes).
  * Nodes are in level order, key order inside a level.  For each node:
  * level, path, particle range [first,last) in sorted order, mass, centre of
- * mass (3), traceless quadrupole about the COM (xx,yy,zz,xy,xz,yz).  Pass
- * capacity = number of slots in the arrays; *count receives the node count. */
+ * mass (3), traceless quadrupole about the COM (xx,yy,zz,xy,xz,yz).  Nodes of
+ * levels 0..7 report their FP64 moments; deeper nodes keep (and report) the
+ * FP32 moments the walk uses, accumulated in FP64 from their children's FP32
+ * records.  Pass capacity = number of slots in the arrays; *count receives
+ * the node count. */
 int nb_force_bh_nodes(nb_force* f, int64_t capacity, int64_t* count, int32_t* level,
                       uint64_t* path, int32_t* first, int32_t* last, double* mass, double* com,
                       double* quad);
@@ -347,10 +350,10 @@ int nb_ensemble_tuning(const nb_ensemble* e, int32_t* shape_codes, int n_buckets
  * an HBM budget, keep their state resident on the device, and every tick
  * advances every active request -- direct-method requests (n <
  * direct_threshold) together in batched launches, K leapfrog steps per tick
- * (K = 4 unless set by nb_server_set_direct_steps, and no more than any of
- * them has left), Barnes-Hut requests B steps each (B = 1 unless set by
- * nb_server_set_bh_steps) on a pool of bh_lanes calculators with their own
- * streams, overlapping on the device.  A request that fails finishes early
+ * (K = 64 unless set by nb_server_set_direct_steps, and no more than any of
+ * them has left), Barnes-Hut requests shortest remaining work first within a
+ * per-tick budget (nb_server_set_tick_budget) on a pool of bh_lanes
+ * calculators with their own streams, overlapping on the device.  A request that fails finishes early
  * with its error status (nb_server_result) without stopping the others.
  * Finished requests are downloaded and released; pending ones are admitted as
  * memory frees up.  Budget accounting: every active request is charged its
@@ -395,13 +398,23 @@ int nb_server_poll_admitted(nb_server* s, int64_t* ids, int capacity, int* count
 int nb_server_result(nb_server* s, int64_t id, double* x, double* y, double* z, double* vx,
                      double* vy, double* vz);
 int nb_server_get_status(const nb_server* s, nb_server_status* out);
-/* Direct-request steps per tick, 1..64 (default 4).  More steps per tick
+/* Direct-request steps per tick, 1..1024 (default 64).  More steps per tick
  * finish small requests sooner while tree steps of large ones share the
  * device; results do not depend on it (each step is the same arithmetic). */
 int nb_server_set_direct_steps(nb_server* s, int steps_per_tick);
-/* Barnes-Hut request steps per tick, 1..64 (default 1).  More steps per tick
- * mean fewer host synchronisations per step; results do not depend on it. */
+/* Barnes-Hut request steps per tick when the tick budget is 0, 1..1024
+ * (default 1).  Results do not depend on it. */
 int nb_server_set_bh_steps(nb_server* s, int steps_per_tick);
+/* Barnes-Hut scheduling by remaining work.  particle_steps > 0: each tick
+ * takes the Barnes-Hut requests in order of remaining work (n x steps left,
+ * shortest first); each gets as many steps as its share of this budget
+ * (particle_steps / bh_lanes) holds, at least one, and requests are added
+ * until the budget is spent and every lane has one (bh_steps is then not
+ * used), so short requests finish early.  Default 4e7 (about 0.15 s of
+ * device time per tick at 2.7e8 particle-steps/s).  0: every Barnes-Hut
+ * request advances up to bh_steps steps every tick.  Results do not depend
+ * on it. */
+int nb_server_set_tick_budget(nb_server* s, double particle_steps);
 
 #ifdef __cplusplus
 }

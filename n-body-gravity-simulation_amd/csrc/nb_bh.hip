@@ -46,6 +46,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdlib>
+#include <cstring>
 
 #include "nb_internal.hpp"
 
@@ -63,6 +64,13 @@ constexpr int kSortHistKeys = 8;       // keys per thread per batch of the digit
 constexpr int kSortHistBlocks = 1024;  // grid cap of the digit-histogram kernel
 constexpr int kSortHistCopies = 8;     // copies of the digit table per histogram block
 constexpr int kMomBatch = 8;  // particles per batch of loads in a leaf's moments
+// Nodes of levels 0..kMomLevels-1 keep an FP64 moment record (mass, centre of
+// mass, quadrupole), from which their parents' moments are accumulated; the
+// nodes below keep only their FP32 walk records, and their parents read
+// those.  Level order numbers the kept nodes first: at most kMomNodes.
+constexpr int kMomLevels = 8;
+constexpr int64_t kMomNodes = 2396745;  // sum of 8^l, l < kMomLevels
+
 constexpr int kUpsweepBlocks = 2048;  // grid cap of a per-level up-sweep launch
 constexpr int kLookbackSpins = 1 << 22;  // polls before a tile gives up waiting (error bit 8)
 constexpr int kLookWindow = 8;  // earlier tiles read per look-back round trip
@@ -135,7 +143,7 @@ struct nb::BHWorkspace {
     DevBuf root, bbox_part, keys0, keys1, vals0, vals1, hist, spos, upos;
     DevBuf lvl_leaf, lvl_sh, blkcnt, blkoff, levels, leaf_of, counters;
     DevBuf n_pstart, n_pend, n_child, n_cend, n_level, n_mom, w_mac, w_link, w_q0, w_q1;
-    DevBuf walk_scratch, gflag, gstart, glev, totals, sortst, gorder;
+    DevBuf walk_scratch, gflag, gstart, glev, totals, sortst, gorder, leaves;
     int walk_blocks = 0;
     int group_size = 0;
 };
@@ -753,8 +761,8 @@ __global__ __launch_bounds__(kBlk) void k_emit(
     const int* __restrict__ blkoff, const LevelInfo* __restrict__ li, int node_cap,
     int* __restrict__ pstart, int* __restrict__ pend, int* __restrict__ child,
     int* __restrict__ cend, signed char* __restrict__ level, int* __restrict__ leaf_of,
-    const unsigned char* __restrict__ gflag, int* __restrict__ gstart, int* __restrict__ gorder,
-    Counters* __restrict__ ctr) {
+    int* __restrict__ leaves, const unsigned char* __restrict__ gflag, int* __restrict__ gstart,
+    int* __restrict__ gorder, Counters* __restrict__ ctr) {
     __shared__ int wcnt[4][kCols];         // per-wave counts: nodes started per level, groups
     __shared__ int lbase[kLevels];         // nodes of each level started by earlier blocks
     __shared__ int lstart[kLevels + 1];
@@ -787,6 +795,9 @@ __global__ __launch_bounds__(kBlk) void k_emit(
     const bool gs = gf != 0;
     const int gb = gf - 1;  // size class of the group starting here
     int grank, orank = 0;
+    const bool first_of_leaf = valid && shi < Li;
+    const unsigned long long bls = __ballot(first_of_leaf);
+    if (lane == 0) wcnt[w][kColLeaf] = __popcll(bls);
     {
         const unsigned long long bal = __ballot(gs);
         grank = __popcll(bal & lt);
@@ -809,6 +820,10 @@ __global__ __launch_bounds__(kBlk) void k_emit(
         gorder[o] = base + grank;
     }
     if (li->total > node_cap) return;  // uniform: the pool check failed, nothing to write
+    // Leaves in particle order (the leaf up-sweep reads their particles in
+    // one sweep over the sorted array).
+    int lrank = blkoff[kColLeaf * nblk + blockIdx.x] + __popcll(bls & lt);
+    for (int k = 0; k < w; ++k) lrank += wcnt[k][kColLeaf];
     // rank(l): nodes of level l started before this particle (all blocks).
     auto rank = [&](int l) {
         const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
@@ -834,7 +849,10 @@ __global__ __launch_bounds__(kBlk) void k_emit(
                     pend[id] = i + 1;
                     cend[id] = (l < Li) ? lstart[l + 1] + rn + (shi <= l ? 1 : 0) : -1;
                 }
-                if (l == Li) leaf_of[i] = id;
+                if (l == Li) {
+                    leaf_of[i] = id;
+                    if (first_of_leaf) leaves[lrank] = id;
+                }
             }
         }
         rl = rn;
@@ -929,15 +947,31 @@ __device__ __forceinline__ void node_moments(int id, int n, int total, const Roo
             }
         }
     } else {
-        // At most 8 children (checked above): one batch.  A record is
-        // (M, cx | cy, cz | Qxx, Qyy | Qzz, Qxy | Qxz, Qyz), 16-byte aligned.
+        // At most 8 children (checked above): one batch.  Children of the
+        // top levels have FP64 records (M, cx | cy, cz | Qxx, Qyy | Qzz, Qxy |
+        // Qxz, Qyz), 16-byte aligned; deeper children only their FP32 walk
+        // records, read into the same form.
         const int c1 = cend[id];
         double2 rc[8][5];
+        if (level[id] + 1 < kMomLevels) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const double2* r = reinterpret_cast<const double2*>(mom + (size_t)min(c0 + k, c1 - 1) * 10);
+            for (int k = 0; k < 8; ++k) {
+                const double2* r =
+                    reinterpret_cast<const double2*>(mom + (size_t)min(c0 + k, c1 - 1) * 10);
 #pragma unroll
-            for (int j = 0; j < 5; ++j) rc[k][j] = r[j];
+                for (int j = 0; j < 5; ++j) rc[k][j] = r[j];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int c = min(c0 + k, c1 - 1);
+                const float4 m4 = mac[c], a4 = q0[c], b4 = q1[c];
+                rc[k][0] = make_double2(a4.x, m4.x);
+                rc[k][1] = make_double2(m4.y, m4.z);
+                rc[k][2] = make_double2(a4.y, a4.z);
+                rc[k][3] = make_double2(a4.w, b4.x);
+                rc[k][4] = make_double2(b4.y, b4.z);
+            }
         }
         double sx = 0, sy = 0, sz = 0, ux = 0, uy = 0, uz = 0;
 #pragma unroll
@@ -980,13 +1014,16 @@ __device__ __forceinline__ void node_moments(int id, int n, int total, const Roo
             }
         }
     }
-    double2* mo = reinterpret_cast<double2*>(mom + (size_t)id * 10);
-    mo[0] = make_double2(M, cx);
-    mo[1] = make_double2(cy, cz);
-    mo[2] = make_double2(Q[0], Q[1]);
-    mo[3] = make_double2(Q[2], Q[3]);
-    mo[4] = make_double2(Q[4], Q[5]);
-    const double size = ldexp(root->size, -(int)level[id]);
+    const int lev = level[id];
+    if (lev < kMomLevels) {  // id < kMomNodes: level order numbers these first
+        double2* mo = reinterpret_cast<double2*>(mom + (size_t)id * 10);
+        mo[0] = make_double2(M, cx);
+        mo[1] = make_double2(cy, cz);
+        mo[2] = make_double2(Q[0], Q[1]);
+        mo[3] = make_double2(Q[2], Q[3]);
+        mo[4] = make_double2(Q[4], Q[5]);
+    }
+    const double size = ldexp(root->size, -lev);
     const double r2open = size * size * inv_theta2;
     mac[id] = make_float4((float)cx, (float)cy, (float)cz, (float)r2open);
     link[id] = kLeaf ? make_int2(~pstart[id], pend[id]) : make_int2(c0, cend[id]);
@@ -994,20 +1031,29 @@ __device__ __forceinline__ void node_moments(int id, int n, int total, const Roo
     q1[id] = make_float4((float)Q[3], (float)Q[4], (float)Q[5], 0.f);
 }
 
-// Leaves depend on their particles only: all of them in one launch.
+// Leaves depend on their particles only: all of them in one launch, in
+// particle order (the leaf list of k_emit), so neighbouring threads read
+// neighbouring particle ranges.
 __global__ __launch_bounds__(kBlk) void k_leaf_moments(
     const LevelInfo* __restrict__ li, const RootInfo* __restrict__ root,
     const int* __restrict__ pstart, const int* __restrict__ pend, const int* child,
     const int* cend, const signed char* __restrict__ level, const float4* __restrict__ spos,
     double* mom, float4* mac, int2* link, float4* q0, float4* q1, double inv_theta2, int quad,
-    int n, Counters* ctr) {
+    int n, const int* __restrict__ leaves, Counters* ctr) {
     if (ctr->overflow & kTreeBad) return;
     const int total = li->total;
-    for (int id = blockIdx.x * kBlk + threadIdx.x; id < total; id += gridDim.x * kBlk)
-        if (child[id] < 0)
-            node_moments<true>(id, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
-                         link, q0, q1, inv_theta2, quad, ctr);
+    const int nl = ctr->n_leaves;
+    for (int k = blockIdx.x * kBlk + threadIdx.x; k < nl; k += gridDim.x * kBlk) {
+        const int id = leaves[k];
+        if ((unsigned)id >= (unsigned)total || child[id] >= 0) {
+            raise_flag(ctr, 4);
+            continue;
+        }
+        node_moments<true>(id, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
+                           link, q0, q1, inv_theta2, quad, ctr);
+    }
 }
+
 
 // Internal nodes, level-synchronous: one launch per level, deepest level
 // first, so every child is finished (and visible at the kernel boundary)
@@ -1420,9 +1466,10 @@ int64_t bh_workspace_bytes(int64_t n, int leaf_cap, double theta, int cus) {
     if (n <= 0) return 0;
     const int64_t nblk = (n + kBlk - 1) / kBlk, nsort = (n + kOsTile - 1) / kOsTile;
     const int64_t ncap = node_capacity(n, leaf_cap);
-    const int64_t per_particle = 8 + 8 + 4 + 4 + 16 + 16 + 1 + 1 + 1 + 4 + 1 + 4 + 4;
-    const int64_t per_node = 4 * 4 + 1 + 10 * 8 + 16 + 8 + 16 + 16;
-    return n * per_particle + 4 + ncap * per_node + nsort * kSortBins * kSortPasses * 4 +
+    const int64_t per_particle = 8 + 8 + 4 + 4 + 16 + 16 + 1 + 1 + 1 + 4 + 1 + 4 + 4 + 4;
+    const int64_t per_node = 4 * 4 + 1 + 16 + 8 + 16 + 16;
+    return n * per_particle + 4 + ncap * per_node + std::min<int64_t>(ncap, kMomNodes) * 80 +
+           nsort * kSortBins * kSortPasses * 4 +
            (int64_t)sizeof(SortState) + nblk * kCols * 4 +
            (int64_t)std::max(kSortBins, kCols) * 4 + 1024 * 6 * 8 + 4096 +
            walk_scratch_bytes(theta, cus);
@@ -1480,7 +1527,8 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_TRY(w->n_child.reserve((size_t)ncap * 4));
     NB_TRY(w->n_cend.reserve((size_t)ncap * 4));
     NB_TRY(w->n_level.reserve((size_t)ncap));
-    NB_TRY(w->n_mom.reserve((size_t)ncap * 10 * 8));
+    NB_TRY(w->n_mom.reserve((size_t)std::min<int64_t>(ncap, kMomNodes) * 10 * 8));
+    NB_TRY(w->leaves.reserve((size_t)n * 4));
     NB_TRY(w->w_mac.reserve((size_t)ncap * 16));
     NB_TRY(w->w_link.reserve((size_t)ncap * 8));
     NB_TRY(w->w_q0.reserve((size_t)ncap * 16));
@@ -1537,7 +1585,8 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->lvl_sh.as<signed char>(), nblk, w->blkcnt.as<int>(), li, (int)ncap,
                        w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
                        w->n_cend.as<int>(), w->n_level.as<signed char>(), w->leaf_of.as<int>(),
-                       w->gflag.as<unsigned char>(), w->gstart.as<int>(), w->gorder.as<int>(), ctr);
+                       w->leaves.as<int>(), w->gflag.as<unsigned char>(), w->gstart.as<int>(),
+                       w->gorder.as<int>(), ctr);
     NB_LAUNCH_CHECK();
     if (ev && ev[5]) NB_HIP(hipEventRecord(ev[5], s));
     const double inv_theta2 = theta > 0.0 ? 1.0 / (theta * theta) : INFINITY;
@@ -1546,7 +1595,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->n_cend.as<int>(), w->n_level.as<signed char>(), w->spos.as<float4>(),
                        w->n_mom.as<double>(), w->w_mac.as<float4>(), w->w_link.as<int2>(),
                        w->w_q0.as<float4>(), w->w_q1.as<float4>(), inv_theta2,
-                       multipole == NB_QUADRUPOLE ? 1 : 0, n, ctr);
+                       multipole == NB_QUADRUPOLE ? 1 : 0, n, w->leaves.as<int>(), ctr);
     for (int l = kLevels - 1; l >= 0; --l) {
         // One thread per node: a level has at most min(8^l, n) nodes.  The
         // grid is capped (the kernel strides): deep levels are mostly empty.
@@ -1710,15 +1759,28 @@ int bh_nodes(BHWorkspace* w, hipStream_t s, int64_t capacity, int64_t* count, in
     *count = li.total;
     if (capacity < li.total) return NB_OK;  // caller asked for the count only
     const int64_t nn = li.total;
+    // Nodes of the top kMomLevels levels (numbered first) have FP64 records;
+    // the others are read back from their FP32 walk records.
+    const int64_t n64 = std::min<int64_t>(nn, li.start[kMomLevels]);
     std::vector<signed char> lv(nn);
     std::vector<int> ps(nn), pe(nn);
     std::vector<double> mo((size_t)nn * 10);
+    std::vector<float4> fm(nn), fa(nn), fb(nn);
     std::vector<unsigned long long> k(w->last_n);
     NB_HIP(hipMemcpy(lv.data(), w->n_level.ptr, nn, hipMemcpyDeviceToHost));
     NB_HIP(hipMemcpy(ps.data(), w->n_pstart.ptr, nn * 4, hipMemcpyDeviceToHost));
     NB_HIP(hipMemcpy(pe.data(), w->n_pend.ptr, nn * 4, hipMemcpyDeviceToHost));
-    NB_HIP(hipMemcpy(mo.data(), w->n_mom.ptr, nn * 80, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(mo.data(), w->n_mom.ptr, n64 * 80, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(fm.data(), w->w_mac.ptr, nn * 16, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(fa.data(), w->w_q0.ptr, nn * 16, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(fb.data(), w->w_q1.ptr, nn * 16, hipMemcpyDeviceToHost));
     NB_HIP(hipMemcpy(k.data(), w->keys0.ptr, w->last_n * 8, hipMemcpyDeviceToHost));
+    for (int64_t q = n64; q < nn; ++q) {
+        double* r = &mo[(size_t)q * 10];
+        const double v[10] = {fa[q].x, fm[q].x, fm[q].y, fm[q].z, fa[q].y,
+                              fa[q].z, fa[q].w, fb[q].x, fb[q].y, fb[q].z};
+        for (int j = 0; j < 10; ++j) r[j] = v[j];
+    }
     for (int64_t q = 0; q < nn; ++q) {
         if (ps[q] < 0 || ps[q] >= w->last_n) {
             set_error("Barnes-Hut nodes: malformed tree (node %lld)", (long long)q);

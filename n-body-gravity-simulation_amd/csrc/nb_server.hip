@@ -2,11 +2,12 @@
 //
 // Requests (one N-body system each, with a step count) are admitted against
 // an HBM budget and keep their state resident on the device until they
-// finish.  Every tick advances each Barnes-Hut request by up to B leapfrog
-// steps (B = 1 unless set, no more than it has left) and each direct request
-// by K steps, K = min(the direct steps per tick, 4 unless set, the fewest
-// steps any direct request has left), so that cheap requests are not paced by
-// the tree steps in the same tick:
+// finish.  Every tick advances the Barnes-Hut requests shortest remaining
+// work first within a budget of particle-steps (4e7 unless set; with a budget
+// of 0 every request takes up to B steps, B = 1 unless set) and each direct
+// request by K steps, K = min(the direct steps per tick, 64 unless set, the
+// fewest steps any direct request has left), so that cheap requests are not
+// paced by the tree steps in the same tick:
 //   * direct-method requests (N below a threshold) are stepped together: per
 //     step one batched kick-drift launch, one persistent force launch over all
 //     of them (work items (request, target tile, shape) pulled from an atomic
@@ -38,9 +39,10 @@
 namespace nb {
 
 constexpr int kSrvBlock = 256;
-constexpr int kDirectStepsPerTick = 4;  // default direct-request steps per tick (fewer near their end)
-constexpr int kDirectStepsPerTickMax = 64;
+constexpr int kDirectStepsPerTick = 64;  // default direct-request steps per tick (fewer near their end)
+constexpr int kDirectStepsPerTickMax = 1024;
 constexpr int kBHStepsPerTick = 1;  // default Barnes-Hut request steps per tick
+constexpr double kTickBudget = 4e7;  // default Barnes-Hut particle-steps per tick, shortest first
 
 struct SrvSys {
     const float4* pack;
@@ -209,6 +211,9 @@ struct nb_server {
     std::vector<int64_t> admitted_since_poll;
     int direct_steps = kDirectStepsPerTick;  // direct-request steps per tick
     int bh_steps = kBHStepsPerTick;          // Barnes-Hut request steps per tick
+    // Barnes-Hut particle-steps per tick; > 0 selects shortest remaining work
+    // first (see nb_server_tick), 0 every request each tick.
+    double tick_budget = kTickBudget;
     DevBuf d_flags;  // one error-flag word per Barnes-Hut request stepped in a tick
     // Two launch slots (priming batch, stepping batch) per tick, each with its
     // own device tables and the host staging they are copied from (kept alive
@@ -430,6 +435,31 @@ int nb_server_tick(nb_server* s, int* n_completed) {
             if (!r->have_acc) fresh.push_back(r);
         }
     }
+    // Barnes-Hut steps of this tick.  Without a budget every request takes
+    // up to bh_steps steps.  With one, requests go in order of remaining
+    // work (particles x steps left, then id); each takes as many steps as
+    // its share of the budget (budget / lanes) holds, at least one, and
+    // requests are added until the budget is spent and every lane has one.
+    std::vector<std::pair<Request*, int>> plan;
+    if (s->tick_budget <= 0.0) {
+        for (Request* r : bh) plan.emplace_back(r, std::min(s->bh_steps, r->desc.steps - r->done));
+    } else {
+        auto rem = [](const Request* r) { return r->desc.n * (int64_t)(r->desc.steps - r->done); };
+        std::stable_sort(bh.begin(), bh.end(), [&](const Request* a, const Request* b) {
+            return rem(a) < rem(b) || (rem(a) == rem(b) && a->id < b->id);
+        });
+        const double share = s->tick_budget / s->n_lanes;
+        double spent = 0.0;
+        for (Request* r : bh) {
+            if (spent >= s->tick_budget && (int)plan.size() >= s->n_lanes) break;
+            const int left = r->desc.steps - r->done;
+            const int k = (int)std::max<double>(1.0, std::min<double>(left, std::floor(share / (double)r->desc.n)));
+            plan.emplace_back(r, k);
+            spent += (double)r->desc.n * k;
+        }
+    }
+    bh.clear();
+    for (const auto& pk : plan) bh.push_back(pk.first);
     // Every Barnes-Hut evaluation of request k ORs its device error flags
     // into word k, read back at the tick's synchronisation.
     if (!bh.empty()) {
@@ -452,7 +482,7 @@ int nb_server_tick(nb_server* s, int* n_completed) {
         f->soft2 = r->desc.softening * r->desc.softening;
         f->theta = r->desc.theta;
         f->err_sink = s->d_flags.as<int>() + k;
-        const int nst = std::min(s->bh_steps, r->desc.steps - r->done);
+        const int nst = plan[k].second;
         const int rc = nb_integrator_run(r->integ, f, &r->p, r->desc.dt, nst);
         f->err_sink = nullptr;
         if (rc == NB_OK) r->stepped = nst;
@@ -565,6 +595,15 @@ int nb_server_set_bh_steps(nb_server* s, int steps_per_tick) {
         return NB_ERR_INVALID;
     }
     s->bh_steps = steps_per_tick;
+    return NB_OK;
+}
+
+int nb_server_set_tick_budget(nb_server* s, double particle_steps) {
+    if (!s || !(particle_steps >= 0.0) || !std::isfinite(particle_steps)) {
+        set_error("nb_server_set_tick_budget: budget must be a finite number >= 0");
+        return NB_ERR_INVALID;
+    }
+    s->tick_budget = particle_steps;
     return NB_OK;
 }
 

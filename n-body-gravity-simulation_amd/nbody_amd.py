@@ -152,6 +152,7 @@ def _load():
         "nb_server_get_status": ([_vp, C.POINTER(_ServerStatus)], C.c_int),
         "nb_server_set_direct_steps": ([_vp, C.c_int], C.c_int),
         "nb_server_set_bh_steps": ([_vp, C.c_int], C.c_int),
+        "nb_server_set_tick_budget": ([_vp, C.c_double], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -617,7 +618,8 @@ class Server:
     """Continuous-batching server (see nb_server_* in include/nbody_amd.h)."""
 
     def __init__(self, hbm_budget_bytes=200e9, direct_threshold=32768, bh_lanes=8,
-                 leaf_capacity=8, direct_steps_per_tick=None, bh_steps_per_tick=None):
+                 leaf_capacity=8, direct_steps_per_tick=None, bh_steps_per_tick=None,
+                 tick_budget=None):
         h = C.c_void_p()
         _check(lib.nb_server_create(C.byref(h), float(hbm_budget_bytes), int(direct_threshold),
                                     int(bh_lanes), int(leaf_capacity)))
@@ -627,6 +629,8 @@ class Server:
             _check(lib.nb_server_set_direct_steps(h, int(direct_steps_per_tick)))
         if bh_steps_per_tick is not None:
             _check(lib.nb_server_set_bh_steps(h, int(bh_steps_per_tick)))
+        if tick_budget is not None:
+            _check(lib.nb_server_set_tick_budget(h, float(tick_budget)))
 
     def submit(self, system, steps, G=1.0, softening=0.005, dt=1e-4, theta=0.5):
         arr = [_f64(system[k]) for k in ("m", "x", "y", "z", "vx", "vy", "vz")]

@@ -103,6 +103,56 @@ def test_moments_closed_form(nb, fixtures):
         assert np.abs(np.array(q, dtype=float) - nodes["quad"][k]).max() <= 1e-10 * float(tot) * s * s
 
 
+def test_moments_deep_tree(nb):
+    """A deep tree (Plummer, 20,000 particles, 12 levels).  Nodes of levels
+    0..7 are accumulated and kept in FP64: to 1e-12 (as above) when their
+    whole subtree lies in those levels; otherwise their inputs include level-8
+    children kept in FP32, so they hold to FP32 rounding of those inputs
+    (u = 6e-8 per stored level down to the leaves: mass 2u, centre
+    2u (|c| + s), quadrupole 30u M s (s + |c|)).
+    Deeper nodes carry the walk's FP32 moments, accumulated in FP64 from
+    their children's FP32 records, to the same kind of bar.  A wrong term,
+    sign or shift would be off by O(M s^2)."""
+    pl = nb.ic_plummer(20000, 14)
+    f, _ = run_bh(nb, pl, eps=0.005, cap=8, G=1.0)
+    top = f.topology(20000)
+    nodes = f.nodes()
+    assert nodes["level"].max() >= 10
+    perm = top["perm"]
+    P = np.stack([pl["x"], pl["y"], pl["z"]], 1)[perm].astype(np.float32).astype(np.longdouble)
+    M = pl["m"][perm].astype(np.float32).astype(np.longdouble)
+    depth = top["depth"][perm]  # leaf level of each particle, sorted order
+    size0 = f.stats()["root_size"]
+    exact = mixed = 0
+    for k in range(len(nodes["mass"])):
+        a, b = nodes["first"][k], nodes["last"][k]
+        m = M[a:b]
+        tot = m.sum()
+        com = (m[:, None] * P[a:b]).sum(0) / tot
+        dd = P[a:b] - com
+        r2 = (dd ** 2).sum(1)
+        q = [np.sum(m * (3 * dd[:, 0] ** 2 - r2)), np.sum(m * (3 * dd[:, 1] ** 2 - r2)),
+             np.sum(m * (3 * dd[:, 2] ** 2 - r2)), np.sum(m * 3 * dd[:, 0] * dd[:, 1]),
+             np.sum(m * 3 * dd[:, 0] * dd[:, 2]), np.sum(m * 3 * dd[:, 1] * dd[:, 2])]
+        s = size0 * 2.0 ** -nodes["level"][k]
+        c = np.array(com, dtype=float)
+        cm = np.abs(c).max()
+        qe = np.abs(np.array(q, dtype=float) - nodes["quad"][k]).max()
+        if nodes["level"][k] < 8 and depth[a:b].max() < 8:
+            exact += 1
+            assert abs(float(tot) - nodes["mass"][k]) <= 1e-12 * float(tot)
+            assert np.abs(c - nodes["com"][k]).max() <= 1e-12 * s
+            assert qe <= 1e-10 * float(tot) * s * s
+        else:
+            mixed += nodes["level"][k] < 8
+            # FP32 rounding once per stored level between the node and its leaves
+            u = 6e-8 * (2 + depth[a:b].max() - nodes["level"][k])
+            assert abs(float(tot) - nodes["mass"][k]) <= 2 * u * float(tot), k
+            assert np.abs(c - nodes["com"][k]).max() <= 2 * u * (cm + s), k
+            assert qe <= 30 * u * float(tot) * s * (s + cm), k
+    assert exact > 100 and mixed > 10
+
+
 @pytest.mark.parametrize("mp,p99,med", [("quadrupole", 1e-3, 3e-4), ("monopole", 5e-3, 1e-3)])
 def test_accuracy_plummer_65536(nb, mp, p99, med):
     pl = nb.ic_plummer(65536, 3)

@@ -22,9 +22,10 @@ def systems(nb):
     return [nb.ic_plummer(n, seed) for n, _, seed in REQS]
 
 
-def run_server(nb, items, stagger=0, budget=200e9, lanes=2, kdir=None, kbh=None):
+def run_server(nb, items, stagger=0, budget=200e9, lanes=2, kdir=None, kbh=None, tick=None,
+               order=None):
     srv = nb.Server(hbm_budget_bytes=budget, direct_threshold=32768, bh_lanes=lanes,
-                    direct_steps_per_tick=kdir, bh_steps_per_tick=kbh)
+                    direct_steps_per_tick=kdir, bh_steps_per_tick=kbh, tick_budget=tick)
     ids = []
     pending = list(items)
     results = {}
@@ -39,6 +40,8 @@ def run_server(nb, items, stagger=0, budget=200e9, lanes=2, kdir=None, kbh=None)
         ticks += 1
         for rid in srv.poll():
             results[rid] = srv.result(rid)
+            if order is not None:
+                order.append(ids.index(rid))
         assert ticks < 10000
     return [results[i] for i in ids], srv
 
@@ -129,7 +132,7 @@ def test_admission_order_and_queueing(nb):
 def test_direct_steps_per_tick_does_not_change_results(nb):
     """Direct requests take K steps per tick (bounded by the fewest steps any
     of them has left); every K gives the same final states, bit for bit, and
-    K outside [1, 64] is rejected."""
+    K outside [1, 1024] is rejected."""
     sy = systems(nb)
     items = [(sy[0], 5), (sy[1], 8), (sy[3], 6), (sy[4], 4)]
     base, _ = run_server(nb, items, stagger=2, kdir=1)
@@ -138,7 +141,7 @@ def test_direct_steps_per_tick_does_not_change_results(nb):
         for a, b in zip(base, res):
             assert all(np.array_equal(a[f], b[f]) for f in FIELDS)
         assert srv.status()["particle_steps"] == sum(len(sd["m"]) * st for sd, st in items)
-    for bad in (0, 65):
+    for bad in (0, 1025):
         with pytest.raises(nb.NBError):
             nb.Server(direct_steps_per_tick=bad)
 
@@ -148,7 +151,7 @@ def test_zero_step_request_and_result_collection(nb):
     state.  Asking for a result too early is an error that keeps the request
     collectable; a result is collected once; an unknown id is an error."""
     sy = systems(nb)
-    srv = nb.Server(direct_threshold=32768, bh_lanes=1)
+    srv = nb.Server(direct_threshold=32768, bh_lanes=1, direct_steps_per_tick=4)
     early = srv.submit(sy[1], 20)  # more steps than one tick takes
     with pytest.raises(nb.NBError) as e:
         srv.result(early)  # still pending
@@ -176,7 +179,9 @@ def test_bh_lane_workspace_accounting(nb):
     n, steps = 3000, 3
     sysd = nb.ic_plummer(n, 21)
     state = n * 80
-    probe = nb.Server(hbm_budget_bytes=200e9, direct_threshold=1000, bh_lanes=2)
+    # one step per tick (no tick budget), so the request is still active
+    # after the first tick
+    probe = nb.Server(hbm_budget_bytes=200e9, direct_threshold=1000, bh_lanes=2, tick_budget=0)
     probe.submit(sysd, steps)
     probe.tick()
     lanes_plus_one = probe.status()["bytes_in_use"]
@@ -190,7 +195,7 @@ def test_bh_lane_workspace_accounting(nb):
 
     # Room for the lanes and one request's state: the second request waits.
     srv = nb.Server(hbm_budget_bytes=lanes_plus_one + state // 2, direct_threshold=1000,
-                    bh_lanes=2)
+                    bh_lanes=2, tick_budget=0)
     ids = [srv.submit(sysd, steps), srv.submit(sysd, steps)]
     results, ticks = {}, 0
     while len(results) < 2:
@@ -211,18 +216,18 @@ def test_bh_lane_workspace_accounting(nb):
 
 
 def test_bh_steps_per_tick_does_not_change_results(nb):
-    """Barnes-Hut requests take up to B steps per tick (fused leapfrog runs);
-    every B gives the same final states, bit for bit, and the particle-step
-    count is exact."""
+    """Without a tick budget, Barnes-Hut requests take up to B steps per tick
+    (fused leapfrog runs); every B gives the same final states, bit for bit,
+    as the default budgeted schedule, and the particle-step count is exact."""
     sy = systems(nb)
     items = [(sy[4], 7), (sy[0], 5), (sy[5], 4), (sy[1], 8)]
-    base, _ = run_server(nb, items, stagger=2, kbh=1)
-    for k in (3, 8):
-        res, srv = run_server(nb, items, stagger=2, kbh=k)
+    base, _ = run_server(nb, items, stagger=2)
+    for k in (1, 3, 8):
+        res, srv = run_server(nb, items, stagger=2, kbh=k, tick=0)
         for a, b in zip(base, res):
             assert all(np.array_equal(a[f], b[f]) for f in FIELDS)
         assert srv.status()["particle_steps"] == sum(len(sd["m"]) * st for sd, st in items)
-    for bad in (0, 65):
+    for bad in (0, 1025):
         with pytest.raises(nb.NBError):
             nb.Server(bh_steps_per_tick=bad)
 
@@ -278,3 +283,21 @@ def test_million_particle_bh_request_equals_library_path(nb):
     for fl in FIELDS:
         assert np.array_equal(g[fl], res[0][fl]), fl
     assert srv.status()["particle_steps"] == n * steps + 2000 * 5
+
+
+def test_shortest_remaining_work_first(nb):
+    """With a per-tick budget the Barnes-Hut requests run shortest remaining
+    work first: submitted together, they finish in order of n x steps, and
+    every final state is bit-identical to the every-request-every-tick
+    schedule; a negative budget is rejected."""
+    sy = systems(nb)
+    items = [(sy[5], 6), (nb.ic_plummer(35000, 9), 2), (sy[4], 4)]  # 300k, 70k, 160k particle-steps
+    base, _ = run_server(nb, items)
+    order = []
+    res, srv = run_server(nb, items, lanes=1, tick=120000, order=order)
+    assert order == [1, 2, 0]
+    for a, b in zip(base, res):
+        assert all(np.array_equal(a[f], b[f]) for f in FIELDS)
+    assert srv.status()["particle_steps"] == sum(len(sd["m"]) * st for sd, st in items)
+    with pytest.raises(nb.NBError):
+        nb.Server(tick_budget=-1.0)
