@@ -151,8 +151,8 @@ def stage_bytes(n, nodes):
     """Algorithmic HBM bytes per force evaluation for each tree stage."""
     return {
         "bbox": 24 * n,                     # FP64 x, y, z read
-        "keys": (32 + 8 + 4 + 16) * n,      # x, y, z, m read; key, index, float4 written
-        "sort": (8 + 8 * 24) * n,           # keys read once for the digit counts; 8 passes x (12 read + 12 write)
+        "keys": (32 + 8 + 4 + 16) * n,      # x, y, z, m read; key, index, float4 written (digit counts in registers)
+        "sort": 8 * 24 * n,                 # 8 passes x (12 read + 12 write)
         "gather": (4 + 16 + 16) * n,        # perm + float4 read through it, float4 written
         "build": 8 * n + 20 * nodes,        # sorted keys read, node topology written
         "upsweep": 16 * n + 2 * 40 * nodes,  # float4 particles + moment records in/out

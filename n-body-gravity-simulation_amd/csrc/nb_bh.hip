@@ -7,9 +7,10 @@
 //             subdivision replayed per particle (child centre = c +/- (s/2)/2,
 //             octant = x>=c | y>=c<<1 | z>=c<<2; same file :159-161,:219-227).
 //             Sorting the keys orders particles by the CPU tree's child order.
-//   sort      LSD radix sort of (key, index), 8 passes of 8 bits, stable: one
-//             digit-count kernel, then one kernel per pass (decoupled look-back
-//             across tiles, tile sorted in LDS, runs written contiguously).
+//   sort      LSD radix sort of (key, index), 8 passes of 8 bits, stable: the
+//             key kernel counts the digits of all passes, then one kernel per
+//             pass (decoupled look-back across tiles, tile sorted in LDS, runs
+//             written contiguously).
 //   gather    sorted float4 (x, y, z, m).
 //   build     k_levels + k_col_scan + k_level_info + k_emit: the tree the reference's
 //             insertion build produces (a cell splits iff it holds more than
@@ -60,9 +61,8 @@ constexpr int kSortPasses = 8;
 constexpr int kSortRows = 16;                    // rows of 64 keys per wave in a radix-pass tile
 constexpr int kOsThreads = 512;                  // threads of a radix-pass block
 constexpr int kOsTile = kOsThreads * kSortRows;  // keys per radix-pass tile
-constexpr int kSortHistKeys = 8;       // keys per thread per batch of the digit-histogram kernel
-constexpr int kSortHistBlocks = 1024;  // grid cap of the digit-histogram kernel
-constexpr int kSortHistCopies = 8;     // copies of the digit table per histogram block
+constexpr int kKeyBlocks = 1024;     // grid cap of the key kernel (it strides)
+constexpr int kKeyHistCopies = 4;    // copies of the digit table per key block
 constexpr int kMomBatch = 8;  // particles per batch of loads in a leaf's moments
 // Nodes of levels 0..kMomLevels-1 keep an FP64 moment record (mass, centre of
 // mass, quadrupole), from which their parents' moments are accumulated; the
@@ -283,8 +283,14 @@ __global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* _
 }
 
 // ------------------------------------------------------------------ keys
-// Also writes the FP32 (x, y, z, m) of every particle in original order, so
-// the gather after the sort moves one 16-byte record per particle.
+// One pass over the FP64 state (grid-stride, at most kKeyBlocks blocks):
+// every particle's 63-bit key and index, its FP32 (x, y, z, m) in original
+// order (so the gather after the sort moves one 16-byte record per
+// particle), and the digit counts of all eight radix passes.  The counts go
+// into kKeyHistCopies copies of a block table in LDS (lane mod copies, rows
+// padded by one word: the high digits of clustered inputs take few values)
+// and are added to the global counts once per block.  Also zeroes the radix
+// passes' look-back words.
 __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__ x,
                                                const double* __restrict__ y,
                                                const double* __restrict__ z,
@@ -293,47 +299,70 @@ __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__
                                                unsigned long long* __restrict__ keys,
                                                unsigned* __restrict__ vals,
                                                float4* __restrict__ upos,
-                                               Counters* __restrict__ ctr) {
-    const int i = blockIdx.x * kBlk + threadIdx.x;
-    if (i >= n) return;
-    double cx = root->center[0], cy = root->center[1], cz = root->center[2];
-    double s = root->size;
-    const double px = x[i], py = y[i], pz = z[i], pm = m[i];
-    upos[i] = make_float4((float)px, (float)py, (float)pz, (float)pm);
-    // A non-finite position or mass makes the evaluation invalid (error bit
-    // 16): the up-sweep and walk skip it.  One report per wave.
-    const bool bad = !(isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(pm));
-    const unsigned long long bb = __ballot(bad);
-    if (bb && lane_id() == __ffsll((long long)bb) - 1) raise_flag(ctr, kInputBad);
-    unsigned long long k = 0;
+                                               Counters* __restrict__ ctr,
+                                               SortState* __restrict__ ss,
+                                               unsigned* __restrict__ look, int nlook) {
+    constexpr int kRow = kSortBins + 1;
+    __shared__ int h[kKeyHistCopies * kSortPasses * kRow];
+    const int t = threadIdx.x;
+    for (int e = t; e < kKeyHistCopies * kSortPasses * kRow; e += kBlk) h[e] = 0;
+    for (int i = blockIdx.x * kBlk + t; i < nlook; i += gridDim.x * kBlk) look[i] = 0u;
+    __syncthreads();
+    int* hc = h + (t % kKeyHistCopies) * kSortPasses * kRow;
+    const double rcx = root->center[0], rcy = root->center[1], rcz = root->center[2];
+    const double rs = root->size;
+    bool bad = false;
+    for (int i0 = blockIdx.x * kBlk; i0 < n; i0 += gridDim.x * kBlk) {
+        const int i = i0 + t;
+        if (i >= n) break;
+        double cx = rcx, cy = rcy, cz = rcz, s = rs;
+        const double px = x[i], py = y[i], pz = z[i], pm = m[i];
+        upos[i] = make_float4((float)px, (float)py, (float)pz, (float)pm);
+        // A non-finite position or mass makes the evaluation invalid (error
+        // bit 16): the up-sweep and walk skip it.
+        bad = bad || !(isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(pm));
+        unsigned long long k = 0;
 #pragma unroll
-    for (int l = 0; l < kMaxDigits; ++l) {
-        const double half = s / 2.0;
-        const double off = half / 2.0;
-        unsigned d = 0;
-        if (px >= cx) {
-            d |= 1u;
-            cx = cx + off;
-        } else {
-            cx = cx + (-off);
+        for (int l = 0; l < kMaxDigits; ++l) {
+            const double half = s / 2.0;
+            const double off = half / 2.0;
+            unsigned d = 0;
+            if (px >= cx) {
+                d |= 1u;
+                cx = cx + off;
+            } else {
+                cx = cx + (-off);
+            }
+            if (py >= cy) {
+                d |= 2u;
+                cy = cy + off;
+            } else {
+                cy = cy + (-off);
+            }
+            if (pz >= cz) {
+                d |= 4u;
+                cz = cz + off;
+            } else {
+                cz = cz + (-off);
+            }
+            k = (k << 3) | d;
+            s = half;
         }
-        if (py >= cy) {
-            d |= 2u;
-            cy = cy + off;
-        } else {
-            cy = cy + (-off);
-        }
-        if (pz >= cz) {
-            d |= 4u;
-            cz = cz + off;
-        } else {
-            cz = cz + (-off);
-        }
-        k = (k << 3) | d;
-        s = half;
+        keys[i] = k;
+        vals[i] = (unsigned)i;
+#pragma unroll
+        for (int p = 0; p < kSortPasses; ++p) atomicAdd(&hc[p * kRow + (int)((k >> (8 * p)) & 255ull)], 1);
     }
-    keys[i] = k;
-    vals[i] = (unsigned)i;
+    const unsigned long long bb = __ballot(bad);  // one report per wave
+    if (bb && lane_id() == __ffsll((long long)bb) - 1) raise_flag(ctr, kInputBad);
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < kSortPasses; ++p) {
+        int c = 0;
+#pragma unroll
+        for (int q = 0; q < kKeyHistCopies; ++q) c += h[(q * kSortPasses + p) * kRow + t];
+        if (c) atomicAdd(&ss->hist[p][t], c);
+    }
 }
 
 // ------------------------------------------------------------------ radix sort
@@ -342,49 +371,6 @@ __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__
 // once up front; each tile finds the keys of the same digit in earlier tiles
 // by decoupled look-back).
 //
-// k_sort_hist: the digit counts of all eight passes, and zeroes the tiles'
-// look-back words.  Each thread loads kSortHistKeys keys before counting, so
-// a batch costs one memory round trip.  The high digits of clustered inputs
-// take few values, so lanes count into kSortHistCopies copies of the table
-// (lane mod kSortHistCopies; rows padded by one word so that equal digits of
-// different copies sit in different LDS banks) to cut same-address atomics.
-__global__ __launch_bounds__(kBlk) void k_sort_hist(int n, const unsigned long long* __restrict__ keys,
-                                                    SortState* __restrict__ ss,
-                                                    unsigned* __restrict__ look, int nlook) {
-    constexpr int kRow = kSortBins + 1;
-    __shared__ int h[kSortHistCopies * kSortPasses * kRow];
-    const int t = threadIdx.x;
-    for (int e = t; e < kSortHistCopies * kSortPasses * kRow; e += kBlk) h[e] = 0;
-    for (int i = blockIdx.x * kBlk + t; i < nlook; i += gridDim.x * kBlk) look[i] = 0u;
-    __syncthreads();
-    int* hc = h + (t % kSortHistCopies) * kSortPasses * kRow;
-    constexpr int kBatch = kBlk * kSortHistKeys;
-    for (int b0 = blockIdx.x * kBatch; b0 < n; b0 += gridDim.x * kBatch) {
-        unsigned long long k[kSortHistKeys];
-#pragma unroll
-        for (int r = 0; r < kSortHistKeys; ++r) {
-            const int i = b0 + r * kBlk + t;
-            k[r] = i < n ? keys[i] : 0ull;
-        }
-#pragma unroll
-        for (int r = 0; r < kSortHistKeys; ++r) {
-            if (b0 + r * kBlk + t < n) {
-#pragma unroll
-                for (int p = 0; p < kSortPasses; ++p)
-                    atomicAdd(&hc[p * kRow + (int)((k[r] >> (8 * p)) & 255ull)], 1);
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < kSortPasses; ++p) {
-        int c = 0;
-#pragma unroll
-        for (int q = 0; q < kSortHistCopies; ++q) c += h[(q * kSortPasses + p) * kRow + t];
-        if (c) atomicAdd(&ss->hist[p][t], c);
-    }
-}
-
 // Exclusive scan of every column of a column-major count table
 // cnt[c * nrows + r] (one 1024-thread block per column, coalesced 1024-row
 // tiles with a running carry); the column sums go to totals[c].
@@ -395,10 +381,14 @@ __global__ __launch_bounds__(1024) void k_col_scan(int nrows, int* __restrict__ 
     const int w = t >> 6;
     int* col = cnt + (size_t)blockIdx.x * nrows;
     int carry = 0;
-    for (int r0 = 0; r0 < nrows; r0 += 1024) {
-        const int r = r0 + t;
-        const int v = r < nrows ? col[r] : 0;
-        const int incl = wave_incl_scan(v);
+    // 4096 rows per step, four consecutive rows per thread.
+    for (int r0 = 0; r0 < nrows; r0 += 4096) {
+        const int r = r0 + 4 * t;
+        int v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = r + q < nrows ? col[r + q] : 0;
+        const int sum4 = v[0] + v[1] + v[2] + v[3];
+        const int incl = wave_incl_scan(sum4);
         if (lane_id() == 63) shw[w] = incl;
         __syncthreads();
         int before = 0, tile = 0;
@@ -407,7 +397,12 @@ __global__ __launch_bounds__(1024) void k_col_scan(int nrows, int* __restrict__ 
             before += k < w ? sw : 0;
             tile += sw;
         }
-        if (r < nrows) col[r] = carry + before + incl - v;
+        int run = carry + before + incl - sum4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (r + q < nrows) col[r + q] = run;
+            run += v[q];
+        }
         carry += tile;
         __syncthreads();
     }
@@ -1547,15 +1542,13 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlk), 0, s, nbb, w->bbox_part.as<double>(),
                        root, ctr, w->sortst.as<SortState>());
     if (ev && ev[1]) NB_HIP(hipEventRecord(ev[1], s));
-    hipLaunchKernelGGL(k_keys, dim3(nblk), dim3(kBlk), 0, s, n, x, y, z, m, root, k0, v0,
-                       w->upos.as<float4>(), ctr);
-    NB_LAUNCH_CHECK();
-    if (ev && ev[2]) NB_HIP(hipEventRecord(ev[2], s));
     SortState* ss = w->sortst.as<SortState>();
     unsigned* look = w->hist.as<unsigned>();
     const int nlook = nsort * kSortBins * kSortPasses;
-    hipLaunchKernelGGL(k_sort_hist, dim3(std::min(grid_for(n, kBlk * kSortHistKeys), kSortHistBlocks)),
-                       dim3(kBlk), 0, s, n, k0, ss, look, nlook);
+    hipLaunchKernelGGL(k_keys, dim3(std::min(nblk, kKeyBlocks)), dim3(kBlk), 0, s, n, x, y, z, m,
+                       root, k0, v0, w->upos.as<float4>(), ctr, ss, look, nlook);
+    NB_LAUNCH_CHECK();
+    if (ev && ev[2]) NB_HIP(hipEventRecord(ev[2], s));
     for (int p = 0; p < kSortPasses; ++p) {
         hipLaunchKernelGGL(k_onesweep, dim3(nsort), dim3(kOsThreads), 0, s, n, k0, v0, k1, v1, p, ss,
                            look + (size_t)p * nsort * kSortBins, ctr);
