@@ -1,5 +1,5 @@
-// nb_runtime.hip -- error reporting, device selection, particle state and the
-// force-calculator handles of the C ABI.
+// nb_runtime.hip -- device selection, particle state and the force-calculator
+// handles of the C ABI.
 #include <dlfcn.h>
 
 #include <cmath>
@@ -14,17 +14,6 @@
 #include "nb_internal.hpp"
 
 namespace nb {
-
-static thread_local char g_err[1024] = "";
-
-void set_error(const char* fmt, ...) {
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof(g_err), fmt, ap);
-    va_end(ap);
-}
-
-void clear_error() { g_err[0] = 0; }
 
 int DevBuf::reserve(size_t want) {
     if (want <= bytes && ptr) return NB_OK;
@@ -206,8 +195,6 @@ int force_eval(nb_force* f, int64_t n, const double* m, const double* x, const d
 using namespace nb;
 
 extern "C" {
-
-const char* nb_last_error(void) { return nb::g_err; }
 
 const char* nb_version(void) { return "nbody_amd 0.1 (gfx950)"; }
 
