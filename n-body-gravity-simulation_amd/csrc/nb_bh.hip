@@ -1,6 +1,7 @@
 // nb_bh.hip -- Barnes-Hut pipeline on gfx950.
 //
-//   bbox      k_bbox_partial + k_bbox_final: FP64 min/max, root cube exactly as
+//   bbox      k_bbox_partial (per-block FP64 min/max), reduced to the root
+//             cube by every block of the key kernel, exactly as
 //             CPU/src/force_calculator_barnes_hut_optimized.cpp:105-124
 //             (centre = (min+max)/2, size = max extent + 1e-9).
 //   keys      k_keys: 63-bit octant-path keys, 21 levels of the reference's FP64
@@ -11,8 +12,8 @@
 //             key kernel counts the digits of all passes, then one kernel per
 //             pass (decoupled look-back across tiles, tile sorted in LDS, runs
 //             written contiguously).
-//   gather    sorted float4 (x, y, z, m).
-//   build     k_levels + k_col_scan + k_level_info + k_emit: the tree the reference's
+//   gather    sorted float4 (x, y, z, m), written by the last radix pass.
+//   build     k_levels + k_col_scan + k_emit: the tree the reference's
 //             insertion build produces (a cell splits iff it holds more than
 //             leaf_capacity particles and its half size is >= 1e-9), built from
 //             the sorted keys without host synchronisation.  A cell at level l
@@ -58,11 +59,12 @@ constexpr int kLevels = 22;      // node levels 0..21
 constexpr int kBlk = 256;
 constexpr int kSortBins = 256;
 constexpr int kSortPasses = 8;
-constexpr int kSortRows = 16;                    // rows of 64 keys per wave in a radix-pass tile
-constexpr int kOsThreads = 512;                  // threads of a radix-pass block
-constexpr int kOsTile = kOsThreads * kSortRows;  // keys per radix-pass tile
+// Radix-pass tiles: a block of T threads holds R rows of 64 keys per wave,
+// T * R keys.  Small systems take small tiles (more blocks to fill the chip),
+// large ones large tiles (fewer look-back words per key).
 constexpr int kKeyBlocks = 1024;     // grid cap of the key kernel (it strides)
 constexpr int kKeyHistCopies = 4;    // copies of the digit table per key block
+constexpr int kKeyBatch = 4;         // particles per thread per step of the key kernel
 constexpr int kMomBatch = 8;  // particles per batch of loads in a leaf's moments
 // Nodes of levels 0..kMomLevels-1 keep an FP64 moment record (mass, centre of
 // mass, quadrupole), from which their parents' moments are accumulated; the
@@ -72,8 +74,10 @@ constexpr int kMomLevels = 8;
 constexpr int64_t kMomNodes = 2396745;  // sum of 8^l, l < kMomLevels
 
 constexpr int kUpsweepBlocks = 2048;  // grid cap of a per-level up-sweep launch
+constexpr int kBandThreads = 512;     // block of a multi-level up-sweep launch
+constexpr int kDenseLevel = 512;      // nodes that give a level its own up-sweep launch
+constexpr int kBandCache = 8192;      // nodes whose child ranges a multi-level launch keeps in LDS
 constexpr int kLookbackSpins = 1 << 22;  // polls before a tile gives up waiting (error bit 8)
-constexpr int kLookWindow = 8;  // earlier tiles read per look-back round trip
 // Walk frontier capacity per block (opened cells per level).  The frontier of
 // a group grows like theta^-3: 2048 entries at theta = 0.5 (the largest seen on
 // a 2^20 Plummer sphere is ~110), scaled for smaller theta up to kWalkFcapMax.
@@ -146,6 +150,23 @@ struct nb::BHWorkspace {
     DevBuf walk_scratch, gflag, gstart, glev, totals, sortst, gorder, leaves;
     int walk_blocks = 0;
     int group_size = 0;
+    // Node counts per level of a recent evaluation, copied to pinned host
+    // memory on a side stream while the walk runs; they plan the next
+    // evaluation's up-sweep launches (a guide only: any plan is correct).
+    hipStream_t hint_stream = nullptr;
+    hipEvent_t hint_ready = nullptr, hint_done = nullptr;
+    int* hint_host = nullptr;
+    int hint[nb::kLevels] = {};
+    bool have_hint = false, hint_pending = false;
+    ~BHWorkspace() {
+        if (hint_stream) {
+            (void)hipStreamSynchronize(hint_stream);
+            (void)hipStreamDestroy(hint_stream);
+        }
+        if (hint_ready) (void)hipEventDestroy(hint_ready);
+        if (hint_done) (void)hipEventDestroy(hint_done);
+        if (hint_host) (void)hipHostFree(hint_host);
+    }
 };
 
 namespace nb {
@@ -192,11 +213,22 @@ __device__ __forceinline__ void raise_flag(Counters* c, int bits) {
 }
 
 // ------------------------------------------------------------------ bbox
+// Per-block min/max of the positions; block 0 also clears this evaluation's
+// counters and the radix sort's digit counts and tile counters (the key
+// kernel that follows adds to them).
 __global__ __launch_bounds__(kBlk) void k_bbox_partial(int n, const double* __restrict__ x,
                                                        const double* __restrict__ y,
                                                        const double* __restrict__ z,
-                                                       double* __restrict__ part) {
+                                                       double* __restrict__ part,
+                                                       Counters* __restrict__ ctr,
+                                                       SortState* __restrict__ ss) {
     __shared__ double sh[6][kBlk];
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) reset_counters(ctr);
+#pragma unroll
+        for (int p = 0; p < kSortPasses; ++p) ss->hist[p][threadIdx.x] = 0;
+        if (threadIdx.x < kSortPasses) ss->tile[threadIdx.x] = 0;
+    }
     double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
     double mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (int i = blockIdx.x * kBlk + threadIdx.x; i < n; i += gridDim.x * kBlk) {
@@ -227,16 +259,13 @@ __global__ __launch_bounds__(kBlk) void k_bbox_partial(int n, const double* __re
     if (t < 6) part[blockIdx.x * 6 + t] = sh[t][0];
 }
 
-__global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* __restrict__ part,
-                                                     RootInfo* __restrict__ root,
-                                                     Counters* __restrict__ ctr,
-                                                     SortState* __restrict__ ss) {
-    __shared__ double sh[6][kBlk];
+// Root cube from the nparts partial boxes, computed by every block of the
+// key kernel (the min/max are exact, so every block gets the same root):
+// centre = (min+max)/2, size = largest extent + 1e-9, and the deepest level
+// a node can reach.  Needs sh[6][kBlk] and a block barrier per step.
+__device__ __forceinline__ void reduce_root(int nparts, const double* __restrict__ part,
+                                            double (*sh)[kBlk], RootInfo* r) {
     const int t = threadIdx.x;
-    if (t == 0) reset_counters(ctr);
-#pragma unroll
-    for (int p = 0; p < kSortPasses; ++p) ss->hist[p][t] = 0;
-    if (t < kSortPasses) ss->tile[t] = 0;
     double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (int k = t; k < nparts; k += kBlk) {
 #pragma unroll
@@ -261,14 +290,14 @@ __global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* _
     if (t == 0) {
         double size = 0.0;
         for (int a = 0; a < 3; ++a) {
-            root->minv[a] = sh[a][0];
-            root->maxv[a] = sh[3 + a][0];
-            root->center[a] = (sh[a][0] + sh[3 + a][0]) / 2;
+            r->minv[a] = sh[a][0];
+            r->maxv[a] = sh[3 + a][0];
+            r->center[a] = (sh[a][0] + sh[3 + a][0]) / 2;
             const double ext = sh[3 + a][0] - sh[a][0];
             size = a == 0 ? ext : (ext > size ? ext : size);
         }
         size += 1e-9;
-        root->size = size;
+        r->size = size;
         // A node of size s may split iff s/2 >= 1e-9
         // (force_calculator_barnes_hut_optimized.cpp:208-211).
         int l = 0;
@@ -277,9 +306,10 @@ __global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* _
             s = s / 2.0;
             ++l;
         }
-        root->lmax = l;
-        root->key_limited = (l == kMaxDigits && s / 2.0 >= 1e-9) ? 1 : 0;
+        r->lmax = l;
+        r->key_limited = (l == kMaxDigits && s / 2.0 >= 1e-9) ? 1 : 0;
     }
+    __syncthreads();
 }
 
 // ------------------------------------------------------------------ keys
@@ -294,8 +324,9 @@ __global__ __launch_bounds__(kBlk) void k_bbox_final(int nparts, const double* _
 __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__ x,
                                                const double* __restrict__ y,
                                                const double* __restrict__ z,
-                                               const double* __restrict__ m,
-                                               const RootInfo* __restrict__ root,
+                                               const double* __restrict__ m, int nparts,
+                                               const double* __restrict__ part,
+                                               RootInfo* __restrict__ root,
                                                unsigned long long* __restrict__ keys,
                                                unsigned* __restrict__ vals,
                                                float4* __restrict__ upos,
@@ -303,55 +334,83 @@ __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__
                                                SortState* __restrict__ ss,
                                                unsigned* __restrict__ look, int nlook) {
     constexpr int kRow = kSortBins + 1;
-    __shared__ int h[kKeyHistCopies * kSortPasses * kRow];
+    static_assert(kKeyHistCopies * kSortPasses * kRow * 4 >= 6 * kBlk * 8, "root scratch");
+    __shared__ __align__(16) int h[kKeyHistCopies * kSortPasses * kRow];  // root scratch first
+    __shared__ RootInfo sr;
     const int t = threadIdx.x;
-    for (int e = t; e < kKeyHistCopies * kSortPasses * kRow; e += kBlk) h[e] = 0;
     for (int i = blockIdx.x * kBlk + t; i < nlook; i += gridDim.x * kBlk) look[i] = 0u;
+    reduce_root(nparts, part, reinterpret_cast<double(*)[kBlk]>(h), &sr);  // ends with a barrier
+    if (blockIdx.x == 0 && t == 0) *root = sr;  // for the later stages
+    for (int e = t; e < kKeyHistCopies * kSortPasses * kRow; e += kBlk) h[e] = 0;
     __syncthreads();
     int* hc = h + (t % kKeyHistCopies) * kSortPasses * kRow;
-    const double rcx = root->center[0], rcy = root->center[1], rcz = root->center[2];
-    const double rs = root->size;
-    bool bad = false;
-    for (int i0 = blockIdx.x * kBlk; i0 < n; i0 += gridDim.x * kBlk) {
-        const int i = i0 + t;
-        if (i >= n) break;
-        double cx = rcx, cy = rcy, cz = rcz, s = rs;
-        const double px = x[i], py = y[i], pz = z[i], pm = m[i];
-        upos[i] = make_float4((float)px, (float)py, (float)pz, (float)pm);
-        // A non-finite position or mass makes the evaluation invalid (error
-        // bit 16): the up-sweep and walk skip it.
-        bad = bad || !(isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(pm));
-        unsigned long long k = 0;
+    const double rcx = sr.center[0], rcy = sr.center[1], rcz = sr.center[2];
+    // Child-centre offsets per level: the subdivision halves the size twice
+    // per level (half = s/2, offset = half/2), exact in binary, so the same
+    // for every particle.
+    double off[kMaxDigits];
+    {
+        double sz = sr.size;
 #pragma unroll
         for (int l = 0; l < kMaxDigits; ++l) {
-            const double half = s / 2.0;
-            const double off = half / 2.0;
-            unsigned d = 0;
-            if (px >= cx) {
-                d |= 1u;
-                cx = cx + off;
-            } else {
-                cx = cx + (-off);
-            }
-            if (py >= cy) {
-                d |= 2u;
-                cy = cy + off;
-            } else {
-                cy = cy + (-off);
-            }
-            if (pz >= cz) {
-                d |= 4u;
-                cz = cz + off;
-            } else {
-                cz = cz + (-off);
-            }
-            k = (k << 3) | d;
-            s = half;
+            const double half = sz / 2.0;
+            off[l] = half / 2.0;
+            sz = half;
         }
-        keys[i] = k;
-        vals[i] = (unsigned)i;
+    }
+    bool bad = false;
+    // kKeyBatch particles per thread per step, all loads issued first.
+    for (int i0 = blockIdx.x * kBlk * kKeyBatch; i0 < n; i0 += gridDim.x * kBlk * kKeyBatch) {
+        double px[kKeyBatch], py[kKeyBatch], pz[kKeyBatch], pm[kKeyBatch];
 #pragma unroll
-        for (int p = 0; p < kSortPasses; ++p) atomicAdd(&hc[p * kRow + (int)((k >> (8 * p)) & 255ull)], 1);
+        for (int b = 0; b < kKeyBatch; ++b) {
+            const int i = min(i0 + b * kBlk + t, n - 1);
+            px[b] = x[i];
+            py[b] = y[i];
+            pz[b] = z[i];
+            pm[b] = m[i];
+        }
+#pragma unroll
+        for (int b = 0; b < kKeyBatch; ++b) {
+            const int i = i0 + b * kBlk + t;
+            if (i < n) {
+                upos[i] = make_float4((float)px[b], (float)py[b], (float)pz[b], (float)pm[b]);
+                // A non-finite position or mass makes the evaluation invalid
+                // (error bit 16): the up-sweep and walk skip it.
+                bad = bad || !(isfinite(px[b]) && isfinite(py[b]) && isfinite(pz[b]) &&
+                               isfinite(pm[b]));
+                double cx = rcx, cy = rcy, cz = rcz;
+                unsigned long long k = 0;
+#pragma unroll
+                for (int l = 0; l < kMaxDigits; ++l) {
+                    unsigned d = 0;
+                    if (px[b] >= cx) {
+                        d |= 1u;
+                        cx = cx + off[l];
+                    } else {
+                        cx = cx + (-off[l]);
+                    }
+                    if (py[b] >= cy) {
+                        d |= 2u;
+                        cy = cy + off[l];
+                    } else {
+                        cy = cy + (-off[l]);
+                    }
+                    if (pz[b] >= cz) {
+                        d |= 4u;
+                        cz = cz + off[l];
+                    } else {
+                        cz = cz + (-off[l]);
+                    }
+                    k = (k << 3) | d;
+                }
+                keys[i] = k;
+                vals[i] = (unsigned)i;
+#pragma unroll
+                for (int p = 0; p < kSortPasses; ++p)
+                    atomicAdd(&hc[p * kRow + (int)((k >> (8 * p)) & 255ull)], 1);
+            }
+        }
     }
     const unsigned long long bb = __ballot(bad);  // one report per wave
     if (bb && lane_id() == __ffsll((long long)bb) - 1) raise_flag(ctr, kInputBad);
@@ -419,10 +478,13 @@ __global__ __launch_bounds__(1024) void k_col_scan(int nrows, int* __restrict__ 
 // publishing its own inclusive prefix.  The tile is then sorted by digit in
 // LDS (stable: wave, row, lane order is input order) and each digit's run is
 // written contiguously.  Look-back word: flag << 30 | count (counts < 2^30).
+template <int kOsThreads, int kSortRows, int kLookWindow>
 __global__ __launch_bounds__(kOsThreads) void k_onesweep(
     int n, const unsigned long long* __restrict__ kin, const unsigned* __restrict__ vin,
     unsigned long long* __restrict__ kout, unsigned* __restrict__ vout, int pass,
-    SortState* __restrict__ ss, unsigned* __restrict__ look, Counters* __restrict__ ctr) {
+    SortState* __restrict__ ss, unsigned* __restrict__ look, Counters* __restrict__ ctr,
+    const float4* __restrict__ upos, float4* __restrict__ spos) {
+    constexpr int kOsTile = kOsThreads * kSortRows;
     constexpr int kWaves = kOsThreads / 64;
     constexpr int kPerWave = kOsTile / kWaves;  // kSortRows rows of 64 keys
     __shared__ unsigned long long sk[kOsTile];  // keys sorted by digit, then values
@@ -585,18 +647,27 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
     for (int r = 0; r < kSortRows; ++r)
         if (wbase + r * 64 < n) sv[pos[r]] = val[r];
     __syncthreads();
+    if (spos) {
+        // Last pass: the values are the sorted order, so the particles'
+        // float4 records are gathered here too (spos[k] = upos[perm[k]]).
+        unsigned vv[kSortRows];
+        float4 pv[kSortRows];
+#pragma unroll
+        for (int r = 0; r < kSortRows; ++r) {
+            vv[r] = (unsigned)dst[r] < (unsigned)n ? sv[t + r * kOsThreads] : 0u;
+            pv[r] = upos[vv[r] < (unsigned)n ? vv[r] : 0u];
+        }
+#pragma unroll
+        for (int r = 0; r < kSortRows; ++r)
+            if ((unsigned)dst[r] < (unsigned)n) {
+                vout[dst[r]] = vv[r];
+                spos[dst[r]] = pv[r];
+            }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < kSortRows; ++r)
         if ((unsigned)dst[r] < (unsigned)n) vout[dst[r]] = sv[t + r * kOsThreads];
-}
-
-__global__ __launch_bounds__(kBlk) void k_gather(int n, const unsigned* __restrict__ perm,
-                                                 const float4* __restrict__ upos,
-                                                 float4* __restrict__ spos) {
-    const int k = blockIdx.x * kBlk + threadIdx.x;
-    if (k >= n) return;
-    const unsigned j = perm[k];
-    spos[k] = upos[j < (unsigned)n ? j : 0u];
 }
 
 // ------------------------------------------------------------------ build
@@ -714,53 +785,53 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
         blkcnt[t * nblk + blockIdx.x] = wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
 }
 
-// Level starts/counts, group and leaf totals from the column sums.  The node
-// total is summed in 64 bits: 22 levels of up to n nodes each can pass 2^31.
-// A tree larger than the pool (which is never above 2^28 nodes, the range of
-// the walk's 32-bit record offsets) sets error bit 2; its total is stored
-// clamped so every later stage sees total > node_cap and writes nothing.
-__global__ void k_level_info(int n, const int* __restrict__ totals, LevelInfo* __restrict__ li,
-                             int node_cap, int* __restrict__ gstart, Counters* __restrict__ ctr) {
-    if (threadIdx.x != 0) return;
+// Level starts/counts, group and leaf totals from the column sums (one
+// thread).  The node total is summed in 64 bits: 22 levels of up to n nodes
+// each can pass 2^31.  Returns false for a tree larger than the pool (which
+// is never above 2^28 nodes, the range of the walk's 32-bit record offsets);
+// its total is stored clamped so every later stage sees total > node_cap and
+// writes nothing.
+__device__ bool level_info(const int* tot, int node_cap, LevelInfo* li) {
     long long run = 0;
     int ml = 0;
     for (int l = 0; l < kLevels; ++l) {
         li->start[l] = (int)min(run, (long long)INT_MAX);
-        li->count[l] = totals[l];
-        run += totals[l];
-        if (totals[l] > 0) ml = l;
+        li->count[l] = tot[l];
+        run += tot[l];
+        if (tot[l] > 0) ml = l;
     }
     const int total = (int)min(run, (long long)INT_MAX);
     li->start[kLevels] = total;
     li->total = total;
     li->max_level = ml;
-    li->n_groups = totals[kColGroup];
+    li->n_groups = tot[kColGroup];
     int gs = 0;
     for (int b = 0; b < kGBuckets; ++b) {
         li->gb_start[b] = gs;
-        gs += totals[kColGB + b];
+        gs += tot[kColGB + b];
     }
-    gstart[totals[kColGroup]] = n;
-    ctr->n_leaves = totals[kColLeaf];
-    ctr->oversize = totals[kColOver];
-    if (run > (long long)node_cap) raise_flag(ctr, 2);
+    return run <= (long long)node_cap;
 }
 
 // Writes the node records: particle range, level, child range; and the leaf
 // of every sorted particle.  Particle i starts the nodes of levels shi+1..Li,
 // ends those of levels shn+1..Li (shn: shared levels with i+1) and records
 // its leaf at level Li, so a wave only visits the levels from the smallest of
-// those bounds to its deepest leaf.
+// those bounds to its deepest leaf.  Every block derives the level layout
+// from the column sums; block 0 publishes it (LevelInfo, leaf and group
+// totals, error bit 2 for an overfull pool) for the later stages.
 __global__ __launch_bounds__(kBlk) void k_emit(
     int n, const signed char* __restrict__ Lin, const signed char* __restrict__ shin, int nblk,
-    const int* __restrict__ blkoff, const LevelInfo* __restrict__ li, int node_cap,
-    int* __restrict__ pstart, int* __restrict__ pend, int* __restrict__ child,
+    const int* __restrict__ blkoff, const int* __restrict__ totals, LevelInfo* __restrict__ li,
+    int node_cap, int* __restrict__ pstart, int* __restrict__ pend, int* __restrict__ child,
     int* __restrict__ cend, signed char* __restrict__ level, int* __restrict__ leaf_of,
     int* __restrict__ leaves, const unsigned char* __restrict__ gflag, int* __restrict__ gstart,
     int* __restrict__ gorder, Counters* __restrict__ ctr) {
     __shared__ int wcnt[4][kCols];         // per-wave counts: nodes started per level, groups
     __shared__ int lbase[kLevels];         // nodes of each level started by earlier blocks
-    __shared__ int lstart[kLevels + 1];
+    __shared__ int stot[kCols];
+    __shared__ LevelInfo sli;
+    __shared__ int sfits;
     const int t = threadIdx.x;
     const int w = t >> 6;
     const int lane = lane_id();
@@ -769,9 +840,23 @@ __global__ __launch_bounds__(kBlk) void k_emit(
     const int Li = valid ? (int)Lin[i] : -1;
     const int shi = valid ? (int)shin[i] : kLevels;
     const int shn = (valid && i + 1 < n) ? (int)shin[i + 1] : -1;
+    if (t < kCols) stot[t] = totals[t];
+    __syncthreads();
+    if (t == 0) {
+        const bool fits = level_info(stot, node_cap, &sli);
+        sfits = fits;
+        if (blockIdx.x == 0) {
+            *li = sli;
+            gstart[sli.n_groups] = n;
+            ctr->n_leaves = stot[kColLeaf];
+            ctr->oversize = stot[kColOver];
+            if (!fits) raise_flag(ctr, 2);
+        }
+    }
     if (lane < kCols) wcnt[w][lane] = 0;
-    if (t <= kLevels) lstart[t] = li->start[t];
     if (t < kLevels) lbase[t] = blkoff[t * nblk + blockIdx.x];
+    __syncthreads();
+    const int* lstart = sli.start;
     // The wave's level range (lo, hi].
     int lo = valid ? min(min(shi, shn), Li - 1) : kLevels, hi = Li;
 #pragma unroll
@@ -810,11 +895,11 @@ __global__ __launch_bounds__(kBlk) void k_emit(
         for (int k = 0; k < w; ++k) base += wcnt[k][kLevels];
         gstart[base + grank] = i;
         // walk order: size classes largest first, key order inside a class
-        int o = li->gb_start[gb] + blkoff[(kColGB + gb) * nblk + blockIdx.x] + orank;
+        int o = sli.gb_start[gb] + blkoff[(kColGB + gb) * nblk + blockIdx.x] + orank;
         for (int k = 0; k < w; ++k) o += wcnt[k][kColGB + gb];
         gorder[o] = base + grank;
     }
-    if (li->total > node_cap) return;  // uniform: the pool check failed, nothing to write
+    if (!sfits) return;  // uniform: the pool check failed, nothing to write
     // Leaves in particle order (the leaf up-sweep reads their particles in
     // one sweep over the sorted array).
     int lrank = blkoff[kColLeaf * nblk + blockIdx.x] + __popcll(bls & lt);
@@ -856,29 +941,24 @@ __global__ __launch_bounds__(kBlk) void k_emit(
 
 
 // ------------------------------------------------------------------ up-sweep
-// Moments of node `id` (FP64): a leaf (kLeaf) from its particles, an
-// internal node from its (already finished) children in child order.  Writes
-// the FP64 record and the FP32 walk copies.
+// Moments of node `id` at level `lev` (FP64): a leaf (kLeaf) from its
+// particles [c0, c1), an internal node from its (already finished) children
+// [c0, c1) in child order.  Writes the FP64 record and the FP32 walk copies.
 template <bool kLeaf>
-__device__ __forceinline__ void node_moments(int id, int n, int total, const RootInfo* root,
-                             const int* __restrict__ pstart, const int* __restrict__ pend,
-                             const int* child, const int* cend,
-                             const signed char* __restrict__ level,
-                             const float4* __restrict__ spos, double* mom, float4* mac,
-                             int2* link, float4* q0, float4* q1, double inv_theta2, int quad,
-                             Counters* ctr) {
-    const int c0 = child[id];  // < 0 for a leaf (kLeaf), >= 0 otherwise: the caller checks
+__device__ __forceinline__ void node_moments(int id, int c0, int c1, int lev, int n, int total,
+                                             const RootInfo* root, const float4* __restrict__ spos,
+                                             double* mom, float4* mac, int2* link, float4* q0,
+                                             float4* q1, double inv_theta2, int quad,
+                                             Counters* ctr) {
     // Structural checks: a malformed tree is reported, never dereferenced.
     if constexpr (kLeaf) {
-        const int a = pstart[id], b = pend[id];
-        if (a < 0 || b > n || a >= b) {
+        if (c0 < 0 || c1 > n || c0 >= c1) {
             raise_flag(ctr, 4);
             link[id] = make_int2(~0, 0);
             return;
         }
     } else {
-        const int c1 = cend[id];
-        if (c0 <= id || c1 > total || c0 >= c1 || c1 - c0 > 8) {
+        if (c0 <= id || c1 > total || c0 >= c1 || c1 - c0 > 8 || lev < 0 || lev >= kLevels) {
             raise_flag(ctr, 4);
             link[id] = make_int2(~0, 0);
             return;
@@ -890,7 +970,7 @@ __device__ __forceinline__ void node_moments(int id, int n, int total, const Roo
     double M = 0, cx = 0, cy = 0, cz = 0;
     double Q[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz
     if constexpr (kLeaf) {
-        const int a = pstart[id], b = pend[id];
+        const int a = c0, b = c1;
         double sx = 0, sy = 0, sz = 0, ux = 0, uy = 0, uz = 0;
         for (int q0 = a; q0 < b; q0 += kMomBatch) {
             float4 pb[kMomBatch];
@@ -946,9 +1026,8 @@ __device__ __forceinline__ void node_moments(int id, int n, int total, const Roo
         // top levels have FP64 records (M, cx | cy, cz | Qxx, Qyy | Qzz, Qxy |
         // Qxz, Qyz), 16-byte aligned; deeper children only their FP32 walk
         // records, read into the same form.
-        const int c1 = cend[id];
         double2 rc[8][5];
-        if (level[id] + 1 < kMomLevels) {
+        if (lev + 1 < kMomLevels) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const double2* r =
@@ -1009,7 +1088,6 @@ __device__ __forceinline__ void node_moments(int id, int n, int total, const Roo
             }
         }
     }
-    const int lev = level[id];
     if (lev < kMomLevels) {  // id < kMomNodes: level order numbers these first
         double2* mo = reinterpret_cast<double2*>(mom + (size_t)id * 10);
         mo[0] = make_double2(M, cx);
@@ -1021,7 +1099,7 @@ __device__ __forceinline__ void node_moments(int id, int n, int total, const Roo
     const double size = ldexp(root->size, -lev);
     const double r2open = size * size * inv_theta2;
     mac[id] = make_float4((float)cx, (float)cy, (float)cz, (float)r2open);
-    link[id] = kLeaf ? make_int2(~pstart[id], pend[id]) : make_int2(c0, cend[id]);
+    link[id] = kLeaf ? make_int2(~c0, c1) : make_int2(c0, c1);
     q0[id] = make_float4((float)M, (float)Q[0], (float)Q[1], (float)Q[2]);
     q1[id] = make_float4((float)Q[3], (float)Q[4], (float)Q[5], 0.f);
 }
@@ -1044,7 +1122,7 @@ __global__ __launch_bounds__(kBlk) void k_leaf_moments(
             raise_flag(ctr, 4);
             continue;
         }
-        node_moments<true>(id, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
+        node_moments<true>(id, pstart[id], pend[id], level[id], n, total, root, spos, mom, mac,
                            link, q0, q1, inv_theta2, quad, ctr);
     }
 }
@@ -1065,10 +1143,65 @@ __global__ __launch_bounds__(kBlk) void k_upsweep(
     const int cnt = li->count[l];
     const int st = li->start[l];
     const int total = li->total;
-    for (int k = blockIdx.x * kBlk + threadIdx.x; k < cnt; k += gridDim.x * kBlk)
-        if (child[st + k] >= 0)
-            node_moments<false>(st + k, n, total, root, pstart, pend, child, cend, level, spos, mom, mac,
-                         link, q0, q1, inv_theta2, quad, ctr);
+    for (int k = blockIdx.x * kBlk + threadIdx.x; k < cnt; k += gridDim.x * kBlk) {
+        const int c0 = child[st + k];
+        if (c0 >= 0)
+            node_moments<false>(st + k, c0, cend[st + k], l, n, total, root, spos, mom, mac, link,
+                                q0, q1, inv_theta2, quad, ctr);
+    }
+}
+
+// A run of consecutive sparse levels, lhi down to llo, in one block: the
+// block barrier between levels replaces a kernel boundary (the block's own
+// stores are visible to it after the barrier).
+__global__ __launch_bounds__(kBandThreads) void k_upsweep_band(
+    int lhi, int llo, const LevelInfo* __restrict__ li, const RootInfo* __restrict__ root,
+    const int* __restrict__ pstart, const int* __restrict__ pend, const int* child,
+    const int* cend, const signed char* __restrict__ level, const float4* __restrict__ spos,
+    double* mom, float4* mac, int2* link, float4* q0, float4* q1, double inv_theta2, int quad,
+    int n, Counters* ctr) {
+    // The run's nodes are one id range (level order); their child ranges are
+    // read into LDS once, up front, when the range fits, so a level costs one
+    // round trip to memory (the children's records) instead of three.
+    __shared__ int sc0[kBandCache], sc1[kBandCache];
+    __shared__ int bad;
+    const int t = threadIdx.x;
+    if (t == 0) bad = ctr->overflow & kTreeBad;
+    const int total = li->total;
+    const int r0 = li->start[llo], r1 = li->start[lhi + 1];
+    const bool cached = r1 - r0 <= kBandCache;
+    if (cached) {
+        constexpr int kPer = kBandCache / kBandThreads;
+        int v0[kPer], v1[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {  // every load in flight before the first store
+            const int k = t + j * kBandThreads;
+            v0[j] = k < r1 - r0 ? child[r0 + k] : 0;
+            v1[j] = k < r1 - r0 ? cend[r0 + k] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int k = t + j * kBandThreads;
+            if (k < r1 - r0) {
+                sc0[k] = v0[j];
+                sc1[k] = v1[j];
+            }
+        }
+    }
+    __syncthreads();
+    if (bad) return;
+    for (int l = lhi; l >= llo; --l) {
+        const int cnt = li->count[l];
+        const int st = li->start[l];
+        for (int k = t; k < cnt; k += kBandThreads) {
+            const int id = st + k;
+            const int c0 = cached ? sc0[id - r0] : child[id];
+            if (c0 >= 0)
+                node_moments<false>(id, c0, cached ? sc1[id - r0] : cend[id], l, n, total, root,
+                                    spos, mom, mac, link, q0, q1, inv_theta2, quad, ctr);
+        }
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------------ walk
@@ -1441,6 +1574,25 @@ int64_t node_capacity(int64_t n, int cap) {
     return bound;
 }
 
+// Keys per radix-pass tile for n keys (2048 to 16384) and earlier tiles read
+// per look-back round trip (8, 16 or 32).  NB_SORT_TILE and NB_SORT_WINDOW
+// (tuning) force them.
+int sort_tile(int64_t n) {
+    if (const char* e = std::getenv("NB_SORT_TILE")) {
+        const int t = std::atoi(e);
+        if (t == 2048 || t == 4096 || t == 8192 || t == 16384) return t;
+    }
+    return 4096;
+}
+
+int sort_window(int64_t n) {
+    if (const char* e = std::getenv("NB_SORT_WINDOW")) {
+        const int t = std::atoi(e);
+        if (t == 8 || t == 16 || t == 32) return t;
+    }
+    return 16;
+}
+
 // Walk frontier capacity per block for opening angle theta.  NB_WALK_FCAP
 // (testing) sets it directly, e.g. small enough to exercise the overflow
 // report (error bit 1); kernels stay in bounds at any value >= 1.
@@ -1459,7 +1611,7 @@ int64_t walk_scratch_bytes(double theta, int cus) {
 
 int64_t bh_workspace_bytes(int64_t n, int leaf_cap, double theta, int cus) {
     if (n <= 0) return 0;
-    const int64_t nblk = (n + kBlk - 1) / kBlk, nsort = (n + kOsTile - 1) / kOsTile;
+    const int64_t nblk = (n + kBlk - 1) / kBlk, nsort = (n + sort_tile(n) - 1) / sort_tile(n);
     const int64_t ncap = node_capacity(n, leaf_cap);
     const int64_t per_particle = 8 + 8 + 4 + 4 + 16 + 16 + 1 + 1 + 1 + 4 + 1 + 4 + 4 + 4;
     const int64_t per_node = 4 * 4 + 1 + 16 + 8 + 16 + 16;
@@ -1487,8 +1639,10 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     }
     const int n = (int)n64;
     const int nblk = grid_for(n, kBlk);
-    const int nsort = grid_for(n, kOsTile);
-    const int nbb = std::min(1024, nblk);
+    const int tile = sort_tile(n);
+    const int win = sort_window(n);
+    const int nsort = grid_for(n, tile);
+    const int nbb = std::min(512, nblk);  // partial boxes (every key block reduces them)
     const int64_t ncap = node_capacity(n, leaf_cap);  // <= kMaxNodes
 
     NB_TRY(w->root.reserve(sizeof(RootInfo)));
@@ -1537,29 +1691,46 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     auto* v0 = w->vals0.as<unsigned>();
     auto* v1 = w->vals1.as<unsigned>();
 
-    hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(kBlk), 0, s, n, x, y, z,
-                       w->bbox_part.as<double>());
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlk), 0, s, nbb, w->bbox_part.as<double>(),
-                       root, ctr, w->sortst.as<SortState>());
-    if (ev && ev[1]) NB_HIP(hipEventRecord(ev[1], s));
     SortState* ss = w->sortst.as<SortState>();
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(kBlk), 0, s, n, x, y, z,
+                       w->bbox_part.as<double>(), ctr, ss);
+    if (ev && ev[1]) NB_HIP(hipEventRecord(ev[1], s));
     unsigned* look = w->hist.as<unsigned>();
     const int nlook = nsort * kSortBins * kSortPasses;
-    hipLaunchKernelGGL(k_keys, dim3(std::min(nblk, kKeyBlocks)), dim3(kBlk), 0, s, n, x, y, z, m,
-                       root, k0, v0, w->upos.as<float4>(), ctr, ss, look, nlook);
+    // Key grid: fewer blocks for small n (each block adds its digit table
+    // to the global counts once); NB_KEY_BLOCKS (tuning) overrides.
+    int kblocks = n <= (1 << 22) ? kKeyBlocks / 2 : kKeyBlocks;
+    if (const char* e = std::getenv("NB_KEY_BLOCKS")) kblocks = std::max(1, std::min(4096, std::atoi(e)));
+    hipLaunchKernelGGL(k_keys, dim3(std::min(nblk, kblocks)), dim3(kBlk), 0, s, n, x, y, z, m,
+                       nbb, w->bbox_part.as<double>(), root, k0, v0, w->upos.as<float4>(), ctr, ss,
+                       look, nlook);
     NB_LAUNCH_CHECK();
     if (ev && ev[2]) NB_HIP(hipEventRecord(ev[2], s));
     for (int p = 0; p < kSortPasses; ++p) {
-        hipLaunchKernelGGL(k_onesweep, dim3(nsort), dim3(kOsThreads), 0, s, n, k0, v0, k1, v1, p, ss,
-                           look + (size_t)p * nsort * kSortBins, ctr);
+        unsigned* lp = look + (size_t)p * nsort * kSortBins;
+        float4* gath = p == kSortPasses - 1 ? w->spos.as<float4>() : nullptr;
+#define NB_PASS(T, R, W)                                                                         \
+    hipLaunchKernelGGL((k_onesweep<T, R, W>), dim3(nsort), dim3(T), 0, s, n, k0, v0, k1, v1, p, \
+                       ss, lp, ctr, w->upos.as<float4>(), gath)
+        switch (tile * 64 + win) {
+            case 2048 * 64 + 32: NB_PASS(256, 8, 32); break;
+            case 4096 * 64 + 16: NB_PASS(512, 8, 16); break;
+            case 4096 * 64 + 32: NB_PASS(512, 8, 32); break;
+            case 8192 * 64 + 16: NB_PASS(512, 16, 16); break;
+            case 8192 * 64 + 32: NB_PASS(512, 16, 32); break;
+            case 16384 * 64 + 8: NB_PASS(1024, 16, 8); break;
+            case 16384 * 64 + 16: NB_PASS(1024, 16, 16); break;
+            case 8192 * 64 + 8: NB_PASS(512, 16, 8); break;
+            default: NB_PASS(512, 8, 16); break;
+        }
+#undef NB_PASS
         std::swap(k0, k1);
         std::swap(v0, v1);
     }
     NB_LAUNCH_CHECK();
     // kSortPasses is even: the sorted keys are back in keys0/vals0.
     if (ev && ev[3]) NB_HIP(hipEventRecord(ev[3], s));
-    hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlk), 0, s, n, v0, w->upos.as<float4>(),
-                       w->spos.as<float4>());
+    // (The gather ran in the last radix pass: the gather stage is empty.)
     if (ev && ev[4]) NB_HIP(hipEventRecord(ev[4], s));
     if (w->group_size == 0) {  // walk group size: NB_WALK_GROUP overrides the default
         w->group_size = kGroup;
@@ -1572,10 +1743,9 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->blkcnt.as<int>(), ctr);
     hipLaunchKernelGGL(k_col_scan, dim3(kCols), dim3(1024), 0, s, nblk, w->blkcnt.as<int>(),
                        w->totals.as<int>());
-    hipLaunchKernelGGL(k_level_info, dim3(1), dim3(64), 0, s, n, w->totals.as<int>(), li,
-                       (int)ncap, w->gstart.as<int>(), ctr);
     hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(kBlk), 0, s, n, w->lvl_leaf.as<signed char>(),
-                       w->lvl_sh.as<signed char>(), nblk, w->blkcnt.as<int>(), li, (int)ncap,
+                       w->lvl_sh.as<signed char>(), nblk, w->blkcnt.as<int>(), w->totals.as<int>(),
+                       li, (int)ncap,
                        w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
                        w->n_cend.as<int>(), w->n_level.as<signed char>(), w->leaf_of.as<int>(),
                        w->leaves.as<int>(), w->gflag.as<unsigned char>(), w->gstart.as<int>(),
@@ -1589,19 +1759,76 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->n_mom.as<double>(), w->w_mac.as<float4>(), w->w_link.as<int2>(),
                        w->w_q0.as<float4>(), w->w_q1.as<float4>(), inv_theta2,
                        multipole == NB_QUADRUPOLE ? 1 : 0, n, w->leaves.as<int>(), ctr);
-    for (int l = kLevels - 1; l >= 0; --l) {
-        // One thread per node: a level has at most min(8^l, n) nodes.  The
-        // grid is capped (the kernel strides): deep levels are mostly empty.
-        const int64_t lvl_max = l < 8 ? std::min<int64_t>((int64_t)1 << (3 * l), n) : n;
-        const int ugrid = (int)std::min<int64_t>(kUpsweepBlocks, std::max<int64_t>(1, (lvl_max + kBlk - 1) / kBlk));
-        hipLaunchKernelGGL(k_upsweep, dim3(ugrid), dim3(kBlk), 0, s, l, li, root,
-                           w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
-                           w->n_cend.as<int>(), w->n_level.as<signed char>(),
-                           w->spos.as<float4>(), w->n_mom.as<double>(), w->w_mac.as<float4>(),
-                           w->w_link.as<int2>(), w->w_q0.as<float4>(), w->w_q1.as<float4>(),
-                           inv_theta2, multipole == NB_QUADRUPOLE ? 1 : 0, n, ctr);
+    // Up-sweep schedule from the level sizes of a recent evaluation: a level
+    // with at least kDenseLevel nodes gets its own launch, a run of sparser
+    // levels one single-block launch.  Without sizes every level gets a launch.
+    bool capturing = false;
+    {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        NB_HIP(hipStreamIsCapturing(s, &cs));
+        capturing = cs != hipStreamCaptureStatusNone;
+    }
+    if (w->hint_pending && !capturing) {
+        const hipError_t q = hipEventQuery(w->hint_done);
+        if (q == hipSuccess) {
+            std::memcpy(w->hint, w->hint_host, sizeof(w->hint));
+            w->have_hint = true;
+            w->hint_pending = false;
+        } else if (q == hipErrorNotReady) {
+            (void)hipGetLastError();  // not an error: the copy is still in flight
+        } else {
+            NB_HIP(q);
+        }
+    }
+    // NB_UPSWEEP_PER_LEVEL (testing) keeps one launch per level;
+    // NB_UPSWEEP_DENSE (tuning) sets the level size of a launch of its own.
+    const bool planned = w->have_hint && !capturing && !std::getenv("NB_UPSWEEP_PER_LEVEL");
+    int dense = kDenseLevel;
+    if (const char* e = std::getenv("NB_UPSWEEP_DENSE")) dense = std::max(1, std::atoi(e));
+    for (int l = kLevels - 1; l >= 0;) {
+        if (!planned || w->hint[l] >= dense) {
+            // One thread per node: a level has at most min(8^l, n) nodes.  The
+            // grid is capped (the kernel strides): deep levels are mostly empty.
+            const int64_t lvl_max = l < 8 ? std::min<int64_t>((int64_t)1 << (3 * l), n) : n;
+            const int ugrid = (int)std::min<int64_t>(
+                kUpsweepBlocks, std::max<int64_t>(1, (lvl_max + kBlk - 1) / kBlk));
+            hipLaunchKernelGGL(k_upsweep, dim3(ugrid), dim3(kBlk), 0, s, l, li, root,
+                               w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
+                               w->n_cend.as<int>(), w->n_level.as<signed char>(),
+                               w->spos.as<float4>(), w->n_mom.as<double>(), w->w_mac.as<float4>(),
+                               w->w_link.as<int2>(), w->w_q0.as<float4>(), w->w_q1.as<float4>(),
+                               inv_theta2, multipole == NB_QUADRUPOLE ? 1 : 0, n, ctr);
+            --l;
+        } else {
+            int lo = l;
+            while (lo > 0 && w->hint[lo - 1] < dense) --lo;
+            hipLaunchKernelGGL(k_upsweep_band, dim3(1), dim3(kBandThreads), 0, s, l, lo, li, root,
+                               w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
+                               w->n_cend.as<int>(), w->n_level.as<signed char>(),
+                               w->spos.as<float4>(), w->n_mom.as<double>(), w->w_mac.as<float4>(),
+                               w->w_link.as<int2>(), w->w_q0.as<float4>(), w->w_q1.as<float4>(),
+                               inv_theta2, multipole == NB_QUADRUPOLE ? 1 : 0, n, ctr);
+            l = lo - 1;
+        }
     }
     NB_LAUNCH_CHECK();
+    if (!w->hint_pending && !capturing) {
+        // Level sizes of this evaluation to pinned host memory, off the main
+        // stream (the walk does not wait for the copy).
+        if (!w->hint_stream) {
+            NB_HIP(hipStreamCreateWithFlags(&w->hint_stream, hipStreamNonBlocking));
+            NB_HIP(hipEventCreateWithFlags(&w->hint_ready, hipEventDisableTiming));
+            NB_HIP(hipEventCreateWithFlags(&w->hint_done, hipEventDisableTiming));
+            NB_HIP(hipHostMalloc((void**)&w->hint_host, sizeof(w->hint), hipHostMallocDefault));
+        }
+        NB_HIP(hipEventRecord(w->hint_ready, s));
+        NB_HIP(hipStreamWaitEvent(w->hint_stream, w->hint_ready, 0));
+        NB_HIP(hipMemcpyAsync(w->hint_host,
+                              reinterpret_cast<const char*>(li) + offsetof(LevelInfo, count),
+                              sizeof(w->hint), hipMemcpyDeviceToHost, w->hint_stream));
+        NB_HIP(hipEventRecord(w->hint_done, w->hint_stream));
+        w->hint_pending = true;
+    }
     if (ev && ev[6]) NB_HIP(hipEventRecord(ev[6], s));
 
     const int ngroups = grid_for(n, 16);  // upper bound on walk groups in flight

@@ -234,7 +234,7 @@ def test_small_and_ragged(nb, O, n):
 @pytest.mark.parametrize("n", [4096, 4097, 3 * 4096 + 17, 40000])
 def test_sort_stable_with_repeated_keys_across_tiles(nb, O, n):
     """The radix sort runs one kernel per 8-bit pass over tiles of 4096 keys
-    chained by look-back.  Keys with many repeats (half the particles sit on 64
+    (the default for these sizes) chained by look-back.  Keys with many repeats (half the particles sit on 64
     shared points) spread over one or more tiles, ragged last tile included,
     must come out in the stable order: ties by original index, bit-exact
     against numpy's stable argsort of the oracle keys."""
@@ -251,6 +251,78 @@ def test_sort_stable_with_repeated_keys_across_tiles(nb, O, n):
     assert np.array_equal(top["perm"], np.argsort(keys, kind="stable").astype(np.uint32))
     f.stats()  # raises on any error flag, the sort's look-back wait limit included
     assert all(np.all(np.isfinite(v)) for v in p.acc())
+
+
+def _clustered(n, seed):
+    """Half the particles on 64 shared points (repeated keys), half nearby."""
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(-1.0, 1.0, size=(64, 3))
+    xyz = pts[rng.integers(0, 64, n)]
+    move = rng.random(n) < 0.5
+    xyz[move] += rng.uniform(-0.5, 0.5, size=(int(move.sum()), 3))
+    return dict(m=np.ones(n), x=xyz[:, 0].copy(), y=xyz[:, 1].copy(), z=xyz[:, 2].copy())
+
+
+@pytest.mark.parametrize("tile,window", [(2048, 32), (4096, 16), (4096, 32), (8192, 8), (8192, 16),
+                                         (8192, 32), (16384, 8), (16384, 16)])
+def test_sort_tile_shapes_give_the_stable_order(nb, O, monkeypatch, tile, window):
+    """Every radix-pass tile shape and look-back window (NB_SORT_TILE,
+    NB_SORT_WINDOW) and a key grid smaller than the key count
+    (NB_KEY_BLOCKS) give the stable order, bit-exact, on repeated keys over
+    several tiles with a ragged last tile; the particle records gathered by the
+    last pass follow it."""
+    monkeypatch.setenv("NB_SORT_TILE", str(tile))
+    monkeypatch.setenv("NB_SORT_WINDOW", str(window))
+    monkeypatch.setenv("NB_KEY_BLOCKS", "3")
+    n = 5 * tile + 17
+    d = _clustered(n, tile + window)
+    f, p = run_bh(nb, d, eps=0.01, cap=8, G=1.0)
+    top = f.topology(n)
+    keys = O.keys63(d["x"], d["y"], d["z"])
+    assert np.array_equal(top["keys"], keys)
+    assert np.array_equal(top["perm"], np.argsort(keys, kind="stable").astype(np.uint32))
+    f.stats()
+    monkeypatch.delenv("NB_SORT_TILE")
+    monkeypatch.delenv("NB_SORT_WINDOW")
+    monkeypatch.delenv("NB_KEY_BLOCKS")
+    g, q = run_bh(nb, d, eps=0.01, cap=8, G=1.0)
+    for a, b in zip(p.acc(), q.acc()):
+        assert np.array_equal(a, b)
+
+
+def _moments_and_acc(f, p):
+    nd = f.nodes()
+    return [nd["mass"], nd["com"], nd["quad"]] + list(p.acc())
+
+
+@pytest.mark.parametrize("dense", [None, "1", "1000000000"])
+def test_planned_upsweep_matches_per_level(nb, monkeypatch, dense):
+    """The up-sweep launch plan (levels with many nodes get a launch of their
+    own, runs of sparse levels share a single-block launch) comes from the
+    level sizes of an earlier evaluation.  Any plan, a stale one included,
+    gives bitwise the moments and accelerations of one launch per level."""
+    if dense:
+        monkeypatch.setenv("NB_UPSWEEP_DENSE", dense)
+    a = nb.ic_plummer(200_000, 5)
+    u = nb.ic_uniform_gaussian(200_000, 6)
+    monkeypatch.setenv("NB_UPSWEEP_PER_LEVEL", "1")
+    ref = {}
+    for name, d in (("a", a), ("u", u)):
+        f, p = run_bh(nb, d, eps=0.01, G=1.0)
+        ref[name] = _moments_and_acc(f, p)
+        f.close()
+    monkeypatch.delenv("NB_UPSWEEP_PER_LEVEL")
+    f = nb.BarnesHutForceCalculator(0.01, 0.5, 8, "quadrupole", G=1.0)
+    for name, d in (("a", a), ("a", a), ("u", u), ("u", u), ("a", a)):
+        # the synchronisation lets the level sizes of this evaluation reach
+        # the host before the next one plans its launches
+        p = nb.Particles.from_arrays(d["m"], d["x"], d["y"], d["z"])
+        f.computeAccelerations(p)
+        nb.synchronize()
+        got = _moments_and_acc(f, p)
+        for x, y in zip(got, ref[name]):
+            assert np.array_equal(x, y)
+    f.stats()
 
 
 def test_identical_positions_refuse_to_split(nb, O):
