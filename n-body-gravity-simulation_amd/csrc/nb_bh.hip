@@ -74,9 +74,9 @@ constexpr int kMomLevels = 8;
 constexpr int64_t kMomNodes = 2396745;  // sum of 8^l, l < kMomLevels
 
 constexpr int kUpsweepBlocks = 2048;  // grid cap of a per-level up-sweep launch
-constexpr int kBandThreads = 512;     // block of a multi-level up-sweep launch
+constexpr int kBandThreads = 512;     // block of the single-block launch of a run of sparse levels
 constexpr int kDenseLevel = 512;      // nodes that give a level its own up-sweep launch
-constexpr int kBandCache = 8192;      // nodes whose child ranges a multi-level launch keeps in LDS
+constexpr int kBandCache = 8192;      // nodes whose child ranges that launch keeps in LDS
 constexpr int kLookbackSpins = 1 << 22;  // polls before a tile gives up waiting (error bit 8)
 // Walk frontier capacity per block (opened cells per level).  The frontier of
 // a group grows like theta^-3: 2048 entries at theta = 0.5 (the largest seen on
@@ -150,6 +150,7 @@ struct nb::BHWorkspace {
     DevBuf walk_scratch, gflag, gstart, glev, totals, sortst, gorder, leaves;
     int walk_blocks = 0;
     int group_size = 0;
+    int sort_slots = 0, sort_slots_key = 0;  // radix-pass blocks resident at once, for which shape
     // Node counts per level of a recent evaluation, copied to pinned host
     // memory on a side stream while the walk runs; they plan the next
     // evaluation's up-sweep launches (a guide only: any plan is correct).
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
     int n, const unsigned long long* __restrict__ kin, const unsigned* __restrict__ vin,
     unsigned long long* __restrict__ kout, unsigned* __restrict__ vout, int pass,
     SortState* __restrict__ ss, unsigned* __restrict__ look, Counters* __restrict__ ctr,
-    const float4* __restrict__ upos, float4* __restrict__ spos) {
+    const float4* __restrict__ upos, float4* __restrict__ spos, int resident) {
     constexpr int kOsTile = kOsThreads * kSortRows;
     constexpr int kWaves = kOsThreads / 64;
     constexpr int kPerWave = kOsTile / kWaves;  // kSortRows rows of 64 keys
@@ -496,13 +497,19 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
     const int w = t >> 6;
     const int lane = lane_id();
     const int shift = 8 * pass;
+    // Tile index: with every tile of the grid resident at once (`resident`,
+    // checked by the host against the occupancy) a tile only ever waits for
+    // blocks that hold or will get a slot, so the block index serves;
+    // otherwise tiles are handed out in start order by a ticket.
+    // This pass's global count of digit t (owner threads), loaded up front.
+    const int g = t < kSortBins ? ss->hist[pass][t] : 0;
     if (t == 0) {
-        stile = atomicAdd(&ss->tile[pass], 1);
+        if (!resident) stile = atomicAdd(&ss->tile[pass], 1);
         sbad = 0;
     }
     for (int e = t; e < kWaves * kSortBins; e += kOsThreads) (&cw[0][0])[e] = 0;
-    __syncthreads();
-    const int tile = stile;
+    if (!resident) __syncthreads();
+    const int tile = resident ? (int)blockIdx.x : stile;
     const int base = tile * kOsTile;
     const int tcnt = min(kOsTile, n - base);
     const int wbase = base + w * kPerWave + lane;
@@ -516,6 +523,7 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
         key[r] = i < n ? kin[i] : 0ull;
         val[r] = i < n ? vin[i] : 0u;
     }
+    __syncthreads();  // the zeroed counts (and sbad) are visible
     // Rank within the wave: lanes with the same digit found with eight ballots.
     const unsigned long long lt = lanemask_lt();
 #pragma unroll
@@ -540,7 +548,7 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
     // Thread t < 256 owns digit t: tile count, publication, offsets.
     const bool owner = t < kSortBins;
     int wc[kWaves];
-    int c = 0, ic = 0, ig = 0, g = 0;
+    int c = 0, ic = 0, ig = 0;
     unsigned* my = look + (size_t)tile * kSortBins + (owner ? t : 0);
     if (owner) {
 #pragma unroll
@@ -550,7 +558,6 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
         }
         __hip_atomic_store(my, ((tile == 0 ? 2u : 1u) << 30) | (unsigned)c, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-        g = ss->hist[pass][t];
         ic = wave_incl_scan(c);
         ig = wave_incl_scan(g);
         if (lane == 63) {
@@ -1706,12 +1713,36 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        look, nlook);
     NB_LAUNCH_CHECK();
     if (ev && ev[2]) NB_HIP(hipEventRecord(ev[2], s));
+    if (w->sort_slots_key != tile * 64 + win) {
+        // Radix-pass blocks the whole GPU holds at once for this tile shape.
+        int dev = 0, cus = 0, per_cu = 0;
+        NB_HIP(hipGetDevice(&dev));
+        NB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+#define NB_OCC(T, R, W) \
+    NB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_onesweep<T, R, W>, T, 0))
+        switch (tile * 64 + win) {
+            case 2048 * 64 + 32: NB_OCC(256, 8, 32); break;
+            case 4096 * 64 + 16: NB_OCC(512, 8, 16); break;
+            case 4096 * 64 + 32: NB_OCC(512, 8, 32); break;
+            case 8192 * 64 + 16: NB_OCC(512, 16, 16); break;
+            case 8192 * 64 + 32: NB_OCC(512, 16, 32); break;
+            case 16384 * 64 + 8: NB_OCC(1024, 16, 8); break;
+            case 16384 * 64 + 16: NB_OCC(1024, 16, 16); break;
+            case 8192 * 64 + 8: NB_OCC(512, 16, 8); break;
+            default: NB_OCC(512, 8, 16); break;
+        }
+#undef NB_OCC
+        w->sort_slots = cus * per_cu;
+        if (const char* e = std::getenv("NB_SORT_TICKET")) w->sort_slots = 0;  // testing: always the ticket
+        w->sort_slots_key = tile * 64 + win;
+    }
     for (int p = 0; p < kSortPasses; ++p) {
         unsigned* lp = look + (size_t)p * nsort * kSortBins;
         float4* gath = p == kSortPasses - 1 ? w->spos.as<float4>() : nullptr;
+        const int resident = nsort <= w->sort_slots / 2 ? 1 : 0;  // half: margin for other work
 #define NB_PASS(T, R, W)                                                                         \
     hipLaunchKernelGGL((k_onesweep<T, R, W>), dim3(nsort), dim3(T), 0, s, n, k0, v0, k1, v1, p, \
-                       ss, lp, ctr, w->upos.as<float4>(), gath)
+                       ss, lp, ctr, w->upos.as<float4>(), gath, resident)
         switch (tile * 64 + win) {
             case 2048 * 64 + 32: NB_PASS(256, 8, 32); break;
             case 4096 * 64 + 16: NB_PASS(512, 8, 16); break;
@@ -1787,9 +1818,11 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     if (const char* e = std::getenv("NB_UPSWEEP_DENSE")) dense = std::max(1, std::atoi(e));
     for (int l = kLevels - 1; l >= 0;) {
         if (!planned || w->hint[l] >= dense) {
-            // One thread per node: a level has at most min(8^l, n) nodes.  The
-            // grid is capped (the kernel strides): deep levels are mostly empty.
-            const int64_t lvl_max = l < 8 ? std::min<int64_t>((int64_t)1 << (3 * l), n) : n;
+            // One thread per node: a level has at most min(8^l, n) nodes (or
+            // about as many as it had, when planned).  The grid is capped
+            // (the kernel strides): deep levels are mostly empty.
+            int64_t lvl_max = l < 8 ? std::min<int64_t>((int64_t)1 << (3 * l), n) : n;
+            if (planned) lvl_max = std::min<int64_t>(lvl_max, w->hint[l] + w->hint[l] / 4 + kBlk);
             const int ugrid = (int)std::min<int64_t>(
                 kUpsweepBlocks, std::max<int64_t>(1, (lvl_max + kBlk - 1) / kBlk));
             hipLaunchKernelGGL(k_upsweep, dim3(ugrid), dim3(kBlk), 0, s, l, li, root,

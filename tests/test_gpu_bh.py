@@ -263,17 +263,22 @@ def _clustered(n, seed):
     return dict(m=np.ones(n), x=xyz[:, 0].copy(), y=xyz[:, 1].copy(), z=xyz[:, 2].copy())
 
 
-@pytest.mark.parametrize("tile,window", [(2048, 32), (4096, 16), (4096, 32), (8192, 8), (8192, 16),
-                                         (8192, 32), (16384, 8), (16384, 16)])
-def test_sort_tile_shapes_give_the_stable_order(nb, O, monkeypatch, tile, window):
+@pytest.mark.parametrize("tile,window,ticket", [(2048, 32, 0), (4096, 16, 0), (4096, 16, 1),
+                                                (4096, 32, 0), (8192, 8, 0), (8192, 8, 1),
+                                                (8192, 16, 0), (8192, 32, 0), (16384, 8, 0),
+                                                (16384, 16, 1)])
+def test_sort_tile_shapes_give_the_stable_order(nb, O, monkeypatch, tile, window, ticket):
     """Every radix-pass tile shape and look-back window (NB_SORT_TILE,
-    NB_SORT_WINDOW) and a key grid smaller than the key count
+    NB_SORT_WINDOW), tiles taken by ticket as well as by block index
+    (NB_SORT_TICKET), and a key grid smaller than the key count
     (NB_KEY_BLOCKS) give the stable order, bit-exact, on repeated keys over
     several tiles with a ragged last tile; the particle records gathered by the
     last pass follow it."""
     monkeypatch.setenv("NB_SORT_TILE", str(tile))
     monkeypatch.setenv("NB_SORT_WINDOW", str(window))
     monkeypatch.setenv("NB_KEY_BLOCKS", "3")
+    if ticket:
+        monkeypatch.setenv("NB_SORT_TICKET", "1")
     n = 5 * tile + 17
     d = _clustered(n, tile + window)
     f, p = run_bh(nb, d, eps=0.01, cap=8, G=1.0)
@@ -282,9 +287,8 @@ def test_sort_tile_shapes_give_the_stable_order(nb, O, monkeypatch, tile, window
     assert np.array_equal(top["keys"], keys)
     assert np.array_equal(top["perm"], np.argsort(keys, kind="stable").astype(np.uint32))
     f.stats()
-    monkeypatch.delenv("NB_SORT_TILE")
-    monkeypatch.delenv("NB_SORT_WINDOW")
-    monkeypatch.delenv("NB_KEY_BLOCKS")
+    for k in ("NB_SORT_TILE", "NB_SORT_WINDOW", "NB_KEY_BLOCKS", "NB_SORT_TICKET"):
+        monkeypatch.delenv(k, raising=False)
     g, q = run_bh(nb, d, eps=0.01, cap=8, G=1.0)
     for a, b in zip(p.acc(), q.acc()):
         assert np.array_equal(a, b)
