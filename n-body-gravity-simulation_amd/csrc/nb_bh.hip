@@ -57,8 +57,9 @@ namespace nb {
 constexpr int kMaxDigits = 21;   // octant digits in a key
 constexpr int kLevels = 22;      // node levels 0..21
 constexpr int kBlk = 256;
-constexpr int kSortBins = 256;
-constexpr int kSortPasses = 8;
+constexpr int kDigitBits = 8;                  // radix digit: 8 bits, 8 passes cover the 63-bit keys
+constexpr int kSortBins = 1 << kDigitBits;
+constexpr int kSortPasses = (63 + kDigitBits - 1) / kDigitBits;
 // Radix-pass tiles: a block of T threads holds R rows of 64 keys per wave,
 // T * R keys.  Small systems take small tiles (more blocks to fill the chip),
 // large ones large tiles (fewer look-back words per key).
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(kBlk) void k_bbox_partial(int n, const double* __re
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) reset_counters(ctr);
 #pragma unroll
-        for (int p = 0; p < kSortPasses; ++p) ss->hist[p][threadIdx.x] = 0;
+        for (int e = threadIdx.x; e < kSortPasses * kSortBins; e += kBlk) (&ss->hist[0][0])[e] = 0;
         if (threadIdx.x < kSortPasses) ss->tile[threadIdx.x] = 0;
     }
     double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
@@ -409,19 +410,19 @@ __global__ __launch_bounds__(kBlk) void k_keys(int n, const double* __restrict__
                 vals[i] = (unsigned)i;
 #pragma unroll
                 for (int p = 0; p < kSortPasses; ++p)
-                    atomicAdd(&hc[p * kRow + (int)((k >> (8 * p)) & 255ull)], 1);
+                    atomicAdd(&hc[p * kRow + (int)((k >> (kDigitBits * p)) & (kSortBins - 1))], 1);
             }
         }
     }
     const unsigned long long bb = __ballot(bad);  // one report per wave
     if (bb && lane_id() == __ffsll((long long)bb) - 1) raise_flag(ctr, kInputBad);
     __syncthreads();
-#pragma unroll
-    for (int p = 0; p < kSortPasses; ++p) {
+    for (int e = t; e < kSortPasses * kSortBins; e += kBlk) {
+        const int p = e / kSortBins, d = e % kSortBins;
         int c = 0;
 #pragma unroll
-        for (int q = 0; q < kKeyHistCopies; ++q) c += h[(q * kSortPasses + p) * kRow + t];
-        if (c) atomicAdd(&ss->hist[p][t], c);
+        for (int q = 0; q < kKeyHistCopies; ++q) c += h[(q * kSortPasses + p) * kRow + d];
+        if (c) atomicAdd(&ss->hist[p][d], c);
     }
 }
 
@@ -487,6 +488,7 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
     const float4* __restrict__ upos, float4* __restrict__ spos, int resident) {
     constexpr int kOsTile = kOsThreads * kSortRows;
     constexpr int kWaves = kOsThreads / 64;
+    static_assert(kOsThreads >= kSortBins, "one owner thread per digit");
     constexpr int kPerWave = kOsTile / kWaves;  // kSortRows rows of 64 keys
     __shared__ unsigned long long sk[kOsTile];  // keys sorted by digit, then values
     __shared__ int cw[kWaves][kSortBins];  // per-wave digit counts, then per-wave slot starts
@@ -496,7 +498,7 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
     const int t = threadIdx.x;
     const int w = t >> 6;
     const int lane = lane_id();
-    const int shift = 8 * pass;
+    const int shift = kDigitBits * pass;
     // Tile index: with every tile of the grid resident at once (`resident`,
     // checked by the host against the occupancy) a tile only ever waits for
     // blocks that hold or will get a slot, so the block index serves;
@@ -524,15 +526,15 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
         val[r] = i < n ? vin[i] : 0u;
     }
     __syncthreads();  // the zeroed counts (and sbad) are visible
-    // Rank within the wave: lanes with the same digit found with eight ballots.
+    // Rank within the wave: lanes with the same digit found with one ballot per digit bit.
     const unsigned long long lt = lanemask_lt();
 #pragma unroll
     for (int r = 0; r < kSortRows; ++r) {
         const bool valid = wbase + r * 64 < n;
-        const int d = (int)((key[r] >> shift) & 255ull);
+        const int d = (int)((key[r] >> shift) & (kSortBins - 1));
         unsigned long long peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < kDigitBits; ++b) {
             const unsigned long long bb = __ballot((d >> b) & 1);
             peers &= ((d >> b) & 1) ? bb : ~bb;
         }
@@ -631,7 +633,7 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
 #pragma unroll
     for (int r = 0; r < kSortRows; ++r) {
         if (wbase + r * 64 < n) {
-            pos[r] += cw[w][(int)((key[r] >> shift) & 255ull)];
+            pos[r] += cw[w][(int)((key[r] >> shift) & (kSortBins - 1))];
             sk[pos[r]] = key[r];
         }
     }
@@ -644,7 +646,7 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
         dst[r] = -1;
         if (j < tcnt) {
             const unsigned long long k = sk[j];
-            dst[r] = goff[(int)((k >> shift) & 255ull)] + j;
+            dst[r] = goff[(int)((k >> shift) & (kSortBins - 1))] + j;
             if ((unsigned)dst[r] < (unsigned)n) kout[dst[r]] = k;
         }
     }
@@ -1581,13 +1583,13 @@ int64_t node_capacity(int64_t n, int cap) {
     return bound;
 }
 
-// Keys per radix-pass tile for n keys (2048 to 16384) and earlier tiles read
+// Keys per radix-pass tile for n keys (4096 or 8192) and earlier tiles read
 // per look-back round trip (8, 16 or 32).  NB_SORT_TILE and NB_SORT_WINDOW
 // (tuning) force them.
 int sort_tile(int64_t n) {
     if (const char* e = std::getenv("NB_SORT_TILE")) {
         const int t = std::atoi(e);
-        if (t == 2048 || t == 4096 || t == 8192 || t == 16384) return t;
+        if (t == 4096 || t == 8192) return t;
     }
     return 4096;
 }
@@ -1693,10 +1695,13 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     RootInfo* root = w->root.as<RootInfo>();
     LevelInfo* li = w->levels.as<LevelInfo>();
     Counters* ctr = w->counters.as<Counters>();
-    auto* k0 = w->keys0.as<unsigned long long>();
-    auto* k1 = w->keys1.as<unsigned long long>();
-    auto* v0 = w->vals0.as<unsigned>();
-    auto* v1 = w->vals1.as<unsigned>();
+    // The key kernel writes the buffer pair that makes the last radix pass
+    // write keys0/vals0 (the passes alternate between the pairs).
+    constexpr bool kOddPasses = kSortPasses % 2 == 1;
+    auto* k0 = (kOddPasses ? w->keys1 : w->keys0).as<unsigned long long>();
+    auto* k1 = (kOddPasses ? w->keys0 : w->keys1).as<unsigned long long>();
+    auto* v0 = (kOddPasses ? w->vals1 : w->vals0).as<unsigned>();
+    auto* v1 = (kOddPasses ? w->vals0 : w->vals1).as<unsigned>();
 
     SortState* ss = w->sortst.as<SortState>();
     hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(kBlk), 0, s, n, x, y, z,
@@ -1721,13 +1726,10 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
 #define NB_OCC(T, R, W) \
     NB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_onesweep<T, R, W>, T, 0))
         switch (tile * 64 + win) {
-            case 2048 * 64 + 32: NB_OCC(256, 8, 32); break;
             case 4096 * 64 + 16: NB_OCC(512, 8, 16); break;
             case 4096 * 64 + 32: NB_OCC(512, 8, 32); break;
             case 8192 * 64 + 16: NB_OCC(512, 16, 16); break;
             case 8192 * 64 + 32: NB_OCC(512, 16, 32); break;
-            case 16384 * 64 + 8: NB_OCC(1024, 16, 8); break;
-            case 16384 * 64 + 16: NB_OCC(1024, 16, 16); break;
             case 8192 * 64 + 8: NB_OCC(512, 16, 8); break;
             default: NB_OCC(512, 8, 16); break;
         }
@@ -1744,13 +1746,10 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     hipLaunchKernelGGL((k_onesweep<T, R, W>), dim3(nsort), dim3(T), 0, s, n, k0, v0, k1, v1, p, \
                        ss, lp, ctr, w->upos.as<float4>(), gath, resident)
         switch (tile * 64 + win) {
-            case 2048 * 64 + 32: NB_PASS(256, 8, 32); break;
             case 4096 * 64 + 16: NB_PASS(512, 8, 16); break;
             case 4096 * 64 + 32: NB_PASS(512, 8, 32); break;
             case 8192 * 64 + 16: NB_PASS(512, 16, 16); break;
             case 8192 * 64 + 32: NB_PASS(512, 16, 32); break;
-            case 16384 * 64 + 8: NB_PASS(1024, 16, 8); break;
-            case 16384 * 64 + 16: NB_PASS(1024, 16, 16); break;
             case 8192 * 64 + 8: NB_PASS(512, 16, 8); break;
             default: NB_PASS(512, 8, 16); break;
         }
@@ -1759,7 +1758,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
         std::swap(v0, v1);
     }
     NB_LAUNCH_CHECK();
-    // kSortPasses is even: the sorted keys are back in keys0/vals0.
+    // The sorted keys and permutation are in keys0/vals0 (k0/v0).
     if (ev && ev[3]) NB_HIP(hipEventRecord(ev[3], s));
     // (The gather ran in the last radix pass: the gather stage is empty.)
     if (ev && ev[4]) NB_HIP(hipEventRecord(ev[4], s));

@@ -234,8 +234,9 @@ def test_small_and_ragged(nb, O, n):
 @pytest.mark.parametrize("n", [4096, 4097, 3 * 4096 + 17, 40000])
 def test_sort_stable_with_repeated_keys_across_tiles(nb, O, n):
     """The radix sort runs one kernel per 8-bit pass over tiles of 4096 keys
-    (the default for these sizes) chained by look-back.  Keys with many repeats (half the particles sit on 64
-    shared points) spread over one or more tiles, ragged last tile included,
+    (the default for these sizes) chained by look-back.  Keys with many
+    repeats (half the particles sit on 64 shared points) spread over one or
+    more tiles, ragged last tile included,
     must come out in the stable order: ties by original index, bit-exact
     against numpy's stable argsort of the oracle keys."""
     rng = np.random.default_rng(n)
@@ -263,10 +264,9 @@ def _clustered(n, seed):
     return dict(m=np.ones(n), x=xyz[:, 0].copy(), y=xyz[:, 1].copy(), z=xyz[:, 2].copy())
 
 
-@pytest.mark.parametrize("tile,window,ticket", [(2048, 32, 0), (4096, 16, 0), (4096, 16, 1),
-                                                (4096, 32, 0), (8192, 8, 0), (8192, 8, 1),
-                                                (8192, 16, 0), (8192, 32, 0), (16384, 8, 0),
-                                                (16384, 16, 1)])
+@pytest.mark.parametrize("tile,window,ticket", [(4096, 16, 0), (4096, 16, 1), (4096, 32, 0),
+                                                (8192, 8, 0), (8192, 8, 1), (8192, 16, 0),
+                                                (8192, 32, 1)])
 def test_sort_tile_shapes_give_the_stable_order(nb, O, monkeypatch, tile, window, ticket):
     """Every radix-pass tile shape and look-back window (NB_SORT_TILE,
     NB_SORT_WINDOW), tiles taken by ticket as well as by block index
