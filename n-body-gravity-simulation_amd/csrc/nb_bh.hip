@@ -66,6 +66,7 @@ constexpr int kSortPasses = (63 + kDigitBits - 1) / kDigitBits;
 constexpr int kKeyBlocks = 1024;     // grid cap of the key kernel (it strides)
 constexpr int kKeyHistCopies = 4;    // copies of the digit table per key block
 constexpr int kKeyBatch = 4;         // particles per thread per step of the key kernel
+constexpr int kBuildRowsMax = 4;     // sorted particles per thread in the build kernels, at most
 constexpr int kMomBatch = 8;  // particles per batch of loads in a leaf's moments
 // Nodes of levels 0..kMomLevels-1 keep an FP64 moment record (mass, centre of
 // mass, quadrupole), from which their parents' moments are accumulated; the
@@ -682,11 +683,15 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
 // ------------------------------------------------------------------ build
 // Leaf level L_i and shared prefix with the predecessor sh_i of every sorted
 // particle, the walk-group start flags, and per-block counts of nodes starting
-// at each level (columns 0..21) and of group starts (column 22).
+// at each level (columns 0..21) and of group starts (column 22).  A block
+// takes a tile of kRows rows of kBlk sorted particles, with one key halo and
+// one set of block barriers for the whole tile (more rows: fewer blocks and
+// halos for large n).
 //
 // A walk group is the tree cell of the smallest level that holds at most
 // kGroup particles (or the particle's leaf if that is shallower), so a group
 // is spatially compact; its particles are contiguous in sorted order.
+template <int kRows>
 __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long* __restrict__ keys,
                                                  int cap, int gsz, const RootInfo* __restrict__ root,
                                                  signed char* __restrict__ Lout,
@@ -695,29 +700,32 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
                                                  signed char* __restrict__ glev, int nblk,
                                                  int* __restrict__ blkcnt,
                                                  Counters* __restrict__ ctr) {
-    constexpr int H = kGroup;  // key halo on both sides of the block
-    __shared__ unsigned long long sk[kBlk + 2 * H];
-    __shared__ signed char ss[kBlk + H];
-    __shared__ signed char sg[kBlk + H];
-    __shared__ int wcnt[4][kCols];
+    constexpr int H = kGroup;       // key halo on both sides of the block's tile
+    constexpr int T = kBlk * kRows;  // sorted particles per block, kRows rows of kBlk
+    __shared__ unsigned long long sk[T + 2 * H];
+    __shared__ signed char ss[T + H];
+    __shared__ signed char sg[T + H];
+    __shared__ int wcnt[4][kCols];  // per-wave counts, summed over the rows
     const int t = threadIdx.x;
     const int w = t >> 6;
-    const int b0 = blockIdx.x * kBlk;
-    for (int q = t; q < kBlk + 2 * H; q += kBlk) {
+    const int lane = lane_id();
+    const int b0 = blockIdx.x * T;
+    for (int q = t; q < T + 2 * H; q += kBlk) {
         const int g = b0 - H + q;
         sk[q] = (g >= 0 && g < n) ? keys[g] : 0ull;
     }
+    if (lane < kCols) wcnt[w][lane] = 0;
     __syncthreads();
     // ss[u]: shared levels of keys (j, j+cap) with j = b0 - cap + u;
     // sg[u]: shared levels of keys (j, j+gsz) with j = b0 - gsz + u (gsz <= kGroup).
-    for (int u = t; u < kBlk + H; u += kBlk) {
-        if (u < kBlk + cap) {
+    for (int u = t; u < T + H; u += kBlk) {
+        if (u < T + cap) {
             const int j = b0 - cap + u;
             ss[u] = (j >= 0 && j + cap < n)
                         ? (signed char)shared_levels(sk[H - cap + u], sk[H + u])
                         : (signed char)-1;
         }
-        if (u < kBlk + gsz) {
+        if (u < T + gsz) {
             const int jg = b0 - gsz + u;
             sg[u] = (jg >= 0 && jg + gsz < n)
                         ? (signed char)shared_levels(sk[H - gsz + u], sk[H + u])
@@ -728,65 +736,74 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
     // Window maxima of sg over gsz+1 entries by doubling in place: after step
     // k, sg[u] is the maximum of the original sg[u .. u + 2^k - 1] (entries
     // whose range would pass the end keep a partial value and are never read).
-    const int ng = kBlk + gsz;
+    constexpr int kPer = T / kBlk + 1;  // sg entries per thread (ng < kPer * kBlk)
+    const int ng = T + gsz;
     int kg = 0;
     while ((2 << kg) <= gsz + 1) ++kg;
     for (int k = 0; k < kg; ++k) {
         const int h = 1 << k;
-        int v0 = 0, v1 = 0;
-        const int u1 = t + kBlk;
-        if (t + 2 * h <= ng) v0 = max((int)sg[t], (int)sg[t + h]);
-        if (u1 + 2 * h <= ng) v1 = max((int)sg[u1], (int)sg[u1 + h]);
-        __syncthreads();
-        if (t + 2 * h <= ng) sg[t] = (signed char)v0;
-        if (u1 + 2 * h <= ng) sg[u1] = (signed char)v1;
-        __syncthreads();
-    }
-    const int i = b0 + t;
-    const bool valid = i < n;
-    const int lmax = root->lmax;
-    int Li = -1, shi = 0, mx = -1, gl = -1;
-    if (valid) {
-        for (int q = 0; q <= cap; ++q) mx = max(mx, (int)ss[t + q]);
-        const int mg = max((int)sg[t], (int)sg[t + gsz + 1 - (1 << kg)]);
-        Li = min(lmax, mx + 1);
-        gl = min(Li, mg + 1);
-        shi = (i == 0) ? -1 : shared_levels(sk[H + t - 1], sk[H + t]);
-        Lout[i] = (signed char)Li;
-        shout[i] = (signed char)shi;
-        glev[i] = (signed char)gl;
-    }
-    const bool leaf_start = valid && shi < Li;
-    const bool group_start = valid && shi < gl;
-    int gb = -1;  // size class of the group starting here
-    if (group_start) {
-        // The group is the level-gl cell holding i: i and the particles after
-        // it sharing its first gl octant digits, so it has more than k
-        // particles iff particle i+k shares them.
-        auto longer = [&](int k) {
-            return i + k < n && shared_levels(sk[H + t], sk[H + t + k]) >= gl;
-        };
-        gb = longer(47) ? 0 : longer(31) ? 1 : longer(15) ? 2 : 3;
-    }
-    if (valid) gflag[i] = (unsigned char)(gb + 1);  // 0: no group starts here
-    const bool over = leaf_start && mx + 1 > lmax && root->key_limited;
+        int v[kPer];
 #pragma unroll
-    for (int l = 0; l < kLevels; ++l) {
-        const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
-        if (lane_id() == 0) wcnt[w][l] = __popcll(bal);
+        for (int m = 0; m < kPer; ++m) {
+            const int u = t + m * kBlk;
+            v[m] = u + 2 * h <= ng ? max((int)sg[u], (int)sg[u + h]) : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < kPer; ++m) {
+            const int u = t + m * kBlk;
+            if (u + 2 * h <= ng) sg[u] = (signed char)v[m];
+        }
+        __syncthreads();
     }
-    {
+    const int lmax = root->lmax;
+    const bool key_limited = root->key_limited != 0;
+    for (int r = 0; r < kRows; ++r) {
+        const int o = r * kBlk + t;  // position in the tile
+        const int i = b0 + o;
+        const bool valid = i < n;
+        int Li = -1, shi = 0, mx = -1, gl = -1;
+        if (valid) {
+            for (int q = 0; q <= cap; ++q) mx = max(mx, (int)ss[o + q]);
+            const int mg = max((int)sg[o], (int)sg[o + gsz + 1 - (1 << kg)]);
+            Li = min(lmax, mx + 1);
+            gl = min(Li, mg + 1);
+            shi = (i == 0) ? -1 : shared_levels(sk[H + o - 1], sk[H + o]);
+            Lout[i] = (signed char)Li;
+            shout[i] = (signed char)shi;
+            glev[i] = (signed char)gl;
+        }
+        const bool leaf_start = valid && shi < Li;
+        const bool group_start = valid && shi < gl;
+        int gb = -1;  // size class of the group starting here
+        if (group_start) {
+            // The group is the level-gl cell holding i: i and the particles
+            // after it sharing its first gl octant digits, so it has more
+            // than k particles iff particle i+k shares them.
+            auto longer = [&](int k) {
+                return i + k < n && shared_levels(sk[H + o], sk[H + o + k]) >= gl;
+            };
+            gb = longer(47) ? 0 : longer(31) ? 1 : longer(15) ? 2 : 3;
+        }
+        if (valid) gflag[i] = (unsigned char)(gb + 1);  // 0: no group starts here
+        const bool over = leaf_start && mx + 1 > lmax && key_limited;
+        // Each wave's lane 0 adds its row's counts to the wave's own entries.
+#pragma unroll
+        for (int l = 0; l < kLevels; ++l) {
+            const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
+            if (lane == 0) wcnt[w][l] += __popcll(bal);
+        }
         const unsigned long long bg = __ballot(group_start), bl = __ballot(leaf_start),
                                  bo = __ballot(over);
-        if (lane_id() == 0) {
-            wcnt[w][kColGroup] = __popcll(bg);
-            wcnt[w][kColLeaf] = __popcll(bl);
-            wcnt[w][kColOver] = __popcll(bo);
+        if (lane == 0) {
+            wcnt[w][kColGroup] += __popcll(bg);
+            wcnt[w][kColLeaf] += __popcll(bl);
+            wcnt[w][kColOver] += __popcll(bo);
         }
 #pragma unroll
         for (int b = 0; b < kGBuckets; ++b) {
             const unsigned long long bb = __ballot(gb == b);
-            if (lane_id() == 0) wcnt[w][kColGB + b] = __popcll(bb);
+            if (lane == 0) wcnt[w][kColGB + b] += __popcll(bb);
         }
     }
     __syncthreads();
@@ -829,6 +846,7 @@ __device__ bool level_info(const int* tot, int node_cap, LevelInfo* li) {
 // those bounds to its deepest leaf.  Every block derives the level layout
 // from the column sums; block 0 publishes it (LevelInfo, leaf and group
 // totals, error bit 2 for an overfull pool) for the later stages.
+template <int kRows>
 __global__ __launch_bounds__(kBlk) void k_emit(
     int n, const signed char* __restrict__ Lin, const signed char* __restrict__ shin, int nblk,
     const int* __restrict__ blkoff, const int* __restrict__ totals, LevelInfo* __restrict__ li,
@@ -836,19 +854,14 @@ __global__ __launch_bounds__(kBlk) void k_emit(
     int* __restrict__ cend, signed char* __restrict__ level, int* __restrict__ leaf_of,
     int* __restrict__ leaves, const unsigned char* __restrict__ gflag, int* __restrict__ gstart,
     int* __restrict__ gorder, Counters* __restrict__ ctr) {
-    __shared__ int wcnt[4][kCols];         // per-wave counts: nodes started per level, groups
-    __shared__ int lbase[kLevels];         // nodes of each level started by earlier blocks
+    __shared__ int wcnt[4][kCols];  // per-wave counts of the current row
+    __shared__ int rbase[kCols];    // per column: earlier blocks' and this block's earlier rows' counts
     __shared__ int stot[kCols];
     __shared__ LevelInfo sli;
     __shared__ int sfits;
     const int t = threadIdx.x;
     const int w = t >> 6;
     const int lane = lane_id();
-    const int i = blockIdx.x * kBlk + t;
-    const bool valid = i < n;
-    const int Li = valid ? (int)Lin[i] : -1;
-    const int shi = valid ? (int)shin[i] : kLevels;
-    const int shn = (valid && i + 1 < n) ? (int)shin[i + 1] : -1;
     if (t < kCols) stot[t] = totals[t];
     __syncthreads();
     if (t == 0) {
@@ -862,89 +875,100 @@ __global__ __launch_bounds__(kBlk) void k_emit(
             if (!fits) raise_flag(ctr, 2);
         }
     }
-    if (lane < kCols) wcnt[w][lane] = 0;
-    if (t < kLevels) lbase[t] = blkoff[t * nblk + blockIdx.x];
+    if (t < kCols) rbase[t] = blkoff[t * nblk + blockIdx.x];
     __syncthreads();
     const int* lstart = sli.start;
-    // The wave's level range (lo, hi].
-    int lo = valid ? min(min(shi, shn), Li - 1) : kLevels, hi = Li;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        lo = min(lo, __shfl_xor(lo, d, 64));
-        hi = max(hi, __shfl_xor(hi, d, 64));
-    }
-    lo = max(lo, -1);
     const unsigned long long lt = lanemask_lt();
-    __builtin_amdgcn_wave_barrier();
-    for (int l = lo + 1; l <= hi; ++l) {
-        const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
-        if (lane == 0) wcnt[w][l] = __popcll(bal);
-    }
-    const int gf = valid ? (int)gflag[i] : 0;
-    const bool gs = gf != 0;
-    const int gb = gf - 1;  // size class of the group starting here
-    int grank, orank = 0;
-    const bool first_of_leaf = valid && shi < Li;
-    const unsigned long long bls = __ballot(first_of_leaf);
-    if (lane == 0) wcnt[w][kColLeaf] = __popcll(bls);
-    {
-        const unsigned long long bal = __ballot(gs);
-        grank = __popcll(bal & lt);
-        if (lane == 0) wcnt[w][kLevels] = __popcll(bal);
+    for (int r = 0; r < kRows; ++r) {
+        const int i = blockIdx.x * (kBlk * kRows) + r * kBlk + t;
+        const bool valid = i < n;
+        const int Li = valid ? (int)Lin[i] : -1;
+        const int shi = valid ? (int)shin[i] : kLevels;
+        const int shn = (valid && i + 1 < n) ? (int)shin[i + 1] : -1;
+        if (lane < kCols) wcnt[w][lane] = 0;
+        // The wave's level range (lo, hi].
+        int lo = valid ? min(min(shi, shn), Li - 1) : kLevels, hi = Li;
 #pragma unroll
-        for (int b = 0; b < kGBuckets; ++b) {
-            const unsigned long long bb = __ballot(gb == b);
-            if (gb == b) orank = __popcll(bb & lt);
-            if (lane == 0) wcnt[w][kColGB + b] = __popcll(bb);
+        for (int d = 1; d < 64; d <<= 1) {
+            lo = min(lo, __shfl_xor(lo, d, 64));
+            hi = max(hi, __shfl_xor(hi, d, 64));
         }
-    }
-    __syncthreads();
-    if (gs) {
-        int base = blkoff[kLevels * nblk + blockIdx.x];
-        for (int k = 0; k < w; ++k) base += wcnt[k][kLevels];
-        gstart[base + grank] = i;
-        // walk order: size classes largest first, key order inside a class
-        int o = sli.gb_start[gb] + blkoff[(kColGB + gb) * nblk + blockIdx.x] + orank;
-        for (int k = 0; k < w; ++k) o += wcnt[k][kColGB + gb];
-        gorder[o] = base + grank;
-    }
-    if (!sfits) return;  // uniform: the pool check failed, nothing to write
-    // Leaves in particle order (the leaf up-sweep reads their particles in
-    // one sweep over the sorted array).
-    int lrank = blkoff[kColLeaf * nblk + blockIdx.x] + __popcll(bls & lt);
-    for (int k = 0; k < w; ++k) lrank += wcnt[k][kColLeaf];
-    // rank(l): nodes of level l started before this particle (all blocks).
-    auto rank = [&](int l) {
-        const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
-        int r = lbase[l] + __popcll(bal & lt);
-        for (int k = 0; k < w; ++k) r += wcnt[k][l];
-        return r;
-    };
-    int rl = lo + 1 >= 0 && lo + 1 < kLevels ? rank(lo + 1) : 0;
-    for (int l = max(lo + 1, 0); l <= hi; ++l) {
-        const int rn = l + 1 < kLevels ? rank(l + 1) : 0;  // the same at level l+1
-        if (valid && l <= Li && (l == Li || l > min(shi, shn))) {
-            const bool starts = shi < l;
-            const int id = lstart[l] + rl + (starts ? 1 : 0) - 1;
-            if (id < lstart[l] || id >= lstart[l + 1]) {
-                raise_flag(ctr, 4);
-            } else {
-                if (starts) {
-                    pstart[id] = i;
-                    level[id] = (signed char)l;
-                    child[id] = (l < Li) ? lstart[l + 1] + rn : -1;
-                }
-                if (l > shn) {
-                    pend[id] = i + 1;
-                    cend[id] = (l < Li) ? lstart[l + 1] + rn + (shi <= l ? 1 : 0) : -1;
-                }
-                if (l == Li) {
-                    leaf_of[i] = id;
-                    if (first_of_leaf) leaves[lrank] = id;
-                }
+        lo = max(lo, -1);
+        __builtin_amdgcn_wave_barrier();
+        for (int l = lo + 1; l <= hi; ++l) {
+            const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
+            if (lane == 0) wcnt[w][l] = __popcll(bal);
+        }
+        const int gf = valid ? (int)gflag[i] : 0;
+        const bool gs = gf != 0;
+        const int gb = gf - 1;  // size class of the group starting here
+        int grank, orank = 0;
+        const bool first_of_leaf = valid && shi < Li;
+        const unsigned long long bls = __ballot(first_of_leaf);
+        if (lane == 0) wcnt[w][kColLeaf] = __popcll(bls);
+        {
+            const unsigned long long bal = __ballot(gs);
+            grank = __popcll(bal & lt);
+            if (lane == 0) wcnt[w][kColGroup] = __popcll(bal);
+#pragma unroll
+            for (int b = 0; b < kGBuckets; ++b) {
+                const unsigned long long bb = __ballot(gb == b);
+                if (gb == b) orank = __popcll(bb & lt);
+                if (lane == 0) wcnt[w][kColGB + b] = __popcll(bb);
             }
         }
-        rl = rn;
+        __syncthreads();
+        if (gs) {
+            int base = rbase[kColGroup] + grank;
+            for (int k = 0; k < w; ++k) base += wcnt[k][kColGroup];
+            gstart[base] = i;
+            // walk order: size classes largest first, key order inside a class
+            int o = sli.gb_start[gb] + rbase[kColGB + gb] + orank;
+            for (int k = 0; k < w; ++k) o += wcnt[k][kColGB + gb];
+            gorder[o] = base;
+        }
+        if (sfits) {  // uniform: with the pool check failed, no node is written
+            // Leaves in particle order (the leaf up-sweep reads their
+            // particles in one sweep over the sorted array).
+            int lrank = rbase[kColLeaf] + __popcll(bls & lt);
+            for (int k = 0; k < w; ++k) lrank += wcnt[k][kColLeaf];
+            // rank(l): nodes of level l started before this particle (all blocks).
+            auto rank = [&](int l) {
+                const unsigned long long bal = __ballot(valid && shi < l && l <= Li);
+                int rk = rbase[l] + __popcll(bal & lt);
+                for (int k = 0; k < w; ++k) rk += wcnt[k][l];
+                return rk;
+            };
+            int rl = lo + 1 >= 0 && lo + 1 < kLevels ? rank(lo + 1) : 0;
+            for (int l = max(lo + 1, 0); l <= hi; ++l) {
+                const int rn = l + 1 < kLevels ? rank(l + 1) : 0;  // the same at level l+1
+                if (valid && l <= Li && (l == Li || l > min(shi, shn))) {
+                    const bool starts = shi < l;
+                    const int id = lstart[l] + rl + (starts ? 1 : 0) - 1;
+                    if (id < lstart[l] || id >= lstart[l + 1]) {
+                        raise_flag(ctr, 4);
+                    } else {
+                        if (starts) {
+                            pstart[id] = i;
+                            level[id] = (signed char)l;
+                            child[id] = (l < Li) ? lstart[l + 1] + rn : -1;
+                        }
+                        if (l > shn) {
+                            pend[id] = i + 1;
+                            cend[id] = (l < Li) ? lstart[l + 1] + rn + (shi <= l ? 1 : 0) : -1;
+                        }
+                        if (l == Li) {
+                            leaf_of[i] = id;
+                            if (first_of_leaf) leaves[lrank] = id;
+                        }
+                    }
+                }
+                rl = rn;
+            }
+        }
+        __syncthreads();  // every read of this row's counts is done
+        if (t < kCols) rbase[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        __syncthreads();
     }
 }
 
@@ -1583,6 +1607,16 @@ int64_t node_capacity(int64_t n, int cap) {
     return bound;
 }
 
+// Sorted particles per thread of the build kernels (1, 2 or 4) for n keys.
+// NB_BUILD_ROWS (tuning) forces it.
+int build_rows(int64_t n) {
+    if (const char* e = std::getenv("NB_BUILD_ROWS")) {
+        const int r = std::atoi(e);
+        if (r == 1 || r == 2 || r == 4) return r;
+    }
+    return n <= ((int64_t)1 << 21) ? 2 : kBuildRowsMax;
+}
+
 // Keys per radix-pass tile for n keys (4096 or 8192) and earlier tiles read
 // per look-back round trip (8, 16 or 32).  NB_SORT_TILE and NB_SORT_WINDOW
 // (tuning) force them.
@@ -1648,6 +1682,8 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     }
     const int n = (int)n64;
     const int nblk = grid_for(n, kBlk);
+    const int rows = build_rows(n);
+    const int ntile = grid_for(n, kBlk * rows);  // build blocks
     const int tile = sort_tile(n);
     const int win = sort_window(n);
     const int nsort = grid_for(n, tile);
@@ -1674,7 +1710,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_TRY(w->upos.reserve((size_t)n * 16));
     NB_TRY(w->lvl_leaf.reserve((size_t)n));
     NB_TRY(w->lvl_sh.reserve((size_t)n));
-    NB_TRY(w->blkcnt.reserve((size_t)nblk * kCols * 4));
+    NB_TRY(w->blkcnt.reserve((size_t)ntile * kCols * 4));
     NB_TRY(w->gflag.reserve((size_t)n));
     NB_TRY(w->gstart.reserve((size_t)(n + 1) * 4));
     NB_TRY(w->gorder.reserve((size_t)n * 4));
@@ -1767,19 +1803,29 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
         if (const char* e = std::getenv("NB_WALK_GROUP"))
             w->group_size = std::max(1, std::min(kGroup, std::atoi(e)));
     }
-    hipLaunchKernelGGL(k_levels, dim3(nblk), dim3(kBlk), 0, s, n, k0, leaf_cap, w->group_size, root,
-                       w->lvl_leaf.as<signed char>(), w->lvl_sh.as<signed char>(),
-                       w->gflag.as<unsigned char>(), w->glev.as<signed char>(), nblk,
-                       w->blkcnt.as<int>(), ctr);
-    hipLaunchKernelGGL(k_col_scan, dim3(kCols), dim3(1024), 0, s, nblk, w->blkcnt.as<int>(),
+#define NB_LEVELS(R)                                                                          \
+    hipLaunchKernelGGL(k_levels<R>, dim3(ntile), dim3(kBlk), 0, s, n, k0, leaf_cap,             \
+                       w->group_size, root, w->lvl_leaf.as<signed char>(),                       \
+                       w->lvl_sh.as<signed char>(), w->gflag.as<unsigned char>(),                \
+                       w->glev.as<signed char>(), ntile, w->blkcnt.as<int>(), ctr)
+    if (rows == 4) NB_LEVELS(4);
+    else if (rows == 2) NB_LEVELS(2);
+    else NB_LEVELS(1);
+#undef NB_LEVELS
+    hipLaunchKernelGGL(k_col_scan, dim3(kCols), dim3(1024), 0, s, ntile, w->blkcnt.as<int>(),
                        w->totals.as<int>());
-    hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(kBlk), 0, s, n, w->lvl_leaf.as<signed char>(),
-                       w->lvl_sh.as<signed char>(), nblk, w->blkcnt.as<int>(), w->totals.as<int>(),
-                       li, (int)ncap,
-                       w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),
-                       w->n_cend.as<int>(), w->n_level.as<signed char>(), w->leaf_of.as<int>(),
-                       w->leaves.as<int>(), w->gflag.as<unsigned char>(), w->gstart.as<int>(),
-                       w->gorder.as<int>(), ctr);
+#define NB_EMIT(R)                                                                            \
+    hipLaunchKernelGGL(k_emit<R>, dim3(ntile), dim3(kBlk), 0, s, n,                             \
+                       w->lvl_leaf.as<signed char>(), w->lvl_sh.as<signed char>(), ntile,        \
+                       w->blkcnt.as<int>(), w->totals.as<int>(), li, (int)ncap,                  \
+                       w->n_pstart.as<int>(), w->n_pend.as<int>(), w->n_child.as<int>(),         \
+                       w->n_cend.as<int>(), w->n_level.as<signed char>(), w->leaf_of.as<int>(),  \
+                       w->leaves.as<int>(), w->gflag.as<unsigned char>(), w->gstart.as<int>(),   \
+                       w->gorder.as<int>(), ctr)
+    if (rows == 4) NB_EMIT(4);
+    else if (rows == 2) NB_EMIT(2);
+    else NB_EMIT(1);
+#undef NB_EMIT
     NB_LAUNCH_CHECK();
     if (ev && ev[5]) NB_HIP(hipEventRecord(ev[5], s));
     const double inv_theta2 = theta > 0.0 ? 1.0 / (theta * theta) : INFINITY;

@@ -294,6 +294,31 @@ def test_sort_tile_shapes_give_the_stable_order(nb, O, monkeypatch, tile, window
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("n", [100_003, 300_000])
+def test_build_rows_give_the_same_tree(nb, O, monkeypatch, n):
+    """The build kernels take one, two or four sorted particles per thread
+    (NB_BUILD_ROWS; the default depends on n).  Every choice gives the same
+    nodes, walk groups and accelerations, bit for bit, and the leaf depths and
+    paths of the oracle's tree."""
+    d = nb.ic_plummer(n, 11)
+    got = []
+    for rows in ("1", "2", "4"):
+        monkeypatch.setenv("NB_BUILD_ROWS", rows)
+        f, p = run_bh(nb, d, eps=0.01, G=1.0)
+        nd = f.nodes()
+        got.append([nd["level"], nd["path"], nd["first"], nd["last"], f.stats()["n_groups"]]
+                   + list(p.acc()))
+        f.close()
+    for other in got[1:]:
+        for x, y in zip(got[0], other):
+            assert np.array_equal(x, y)
+    monkeypatch.setenv("NB_BUILD_ROWS", "4")
+    f, p = run_bh(nb, d, eps=0.01, G=1.0)
+    top = f.topology(n)
+    dep, path, _, _ = O.bhopt_leaves(d["m"], d["x"], d["y"], d["z"], 8)
+    assert np.array_equal(top["depth"], dep) and np.array_equal(top["path"], path)
+
+
 def _moments_and_acc(f, p):
     nd = f.nodes()
     return [nd["mass"], nd["com"], nd["quad"]] + list(p.acc())
