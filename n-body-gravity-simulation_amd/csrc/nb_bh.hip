@@ -1343,8 +1343,27 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             fz = fmaf(dz, c, fz);
         };
 
+        // Subset sub evaluates slots sub, sub + nsub, ... of a batch.  Every
+        // subset makes the same number of trips, so the loop counter stays
+        // scalar: a partial last batch is padded to a multiple of nsub with
+        // massless copies of slot 0, which add exact zeros.  Lanes outside the
+        // subsets follow subset 0 and their sums are never read.
+        const int jfirst = live ? sub : 0;
+        auto trips_for = [&](int cnt) {
+            return __builtin_amdgcn_readfirstlane(cnt == cfull ? cfull / nsub
+                                                                : (cnt + nsub - 1) / nsub);
+        };
+
         auto flush_particles = [&](int cnt) {
-            for (int j = live ? sub : cnt; j < cnt; j += nsub) {
+            const int trips = trips_for(cnt);
+            if (trips * nsub > cnt) {  // only a group's last batch: cnt < cfull <= 64
+                float4 v = pl[0];
+                v.w = 0.f;
+                if (lane >= cnt && lane < trips * nsub) pl[lane] = v;
+                __syncthreads();
+            }
+            int j = jfirst;
+            for (int t = 0; t < trips; ++t, j += nsub) {
                 const float4 s = pl[j];
                 pp_one(s, meA, fxA, fyA, fzA);
                 pp_one(s, meB, fxB, fyB, fzB);
@@ -1382,8 +1401,20 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
 
         // Evaluates the cnt staged cells; the staging slots are free afterwards.
         auto flush_cells = [&](int cnt) {
+            const int trips = trips_for(cnt);
+            if (trips * nsub > cnt) {  // only a group's last batch: cnt < cfull <= 64
+                __syncthreads();
+                const float4 v = sc0[0];  // an accepted cell: r^2 > 0 for every target
+                const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (lane >= cnt && lane < trips * nsub) {
+                    sc0[lane] = v;
+                    sc1[lane] = zero;
+                    if (kQuad) sc2[lane] = zero;
+                }
+            }
             __syncthreads();
-            for (int j = live ? sub : cnt; j < cnt; j += nsub) {
+            int j = jfirst;
+            for (int t = 0; t < trips; ++t, j += nsub) {
                 const float4 c = sc0[j];
                 const float4 qa = sc1[j];
                 const float4 qb = kQuad ? sc2[j] : qa;
