@@ -197,6 +197,8 @@ typedef struct nb_bh_stats {
                                   nb_force_bh_stats call (the last one's included); reading
                                   clears it */
     int32_t leaf_capacity;     /* leaf capacity of the last evaluation */
+    int64_t far_cells;         /* (walk group, cell) pairs whose field entered the group's local
+                                  expansion instead of the per-target sum (quadrupole walk) */
 } nb_bh_stats;
 /* Fills *out and returns NB_OK, or (with *out filled) NB_ERR_CAPACITY if an
  * evaluation since the previous call set flag 1, 2, 4 or 8 (its NaN

@@ -89,6 +89,7 @@ constexpr int kWalkFcap = 2048;
 constexpr int kWalkFcapMax = 65536;
 constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
 constexpr int kGroup = 64;      // walk group: cells of at most 64 particles
+constexpr double kWalkFarRatio = 0.3;  // far cells: box half diagonal < 0.3 theta |R|
 constexpr int64_t kMaxNodes = (int64_t)1 << 28;  // node pool limit (32-bit record offsets in the walk)
 // Per-block count columns: 22 levels, group starts, leaf starts, oversize leaves.
 // Walk groups are handed out by size class, largest first (a group's cost
@@ -118,6 +119,7 @@ struct LevelInfo {
 
 struct Counters {
     unsigned long long pp, pc;
+    unsigned long long pf;  // far cells: (group, cell) pairs taken into a group's local expansion
     int max_frontier;
     int overflow;   // this evaluation: bit 0 walk frontier, bit 1 node capacity, bit 2
                     // malformed tree, bit 3 sort look-back wait limit, bit 4 non-finite input
@@ -181,6 +183,22 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
     return (1ull << lane_id()) - 1ull;
 }
 
+// v + (v of the lane the DPP control selects); lanes of rows outside
+// kRowMask add 0.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ float dpp_add(float v) {
+    return v + __int_as_float(
+                   __builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, kRowMask, 0xf, false));
+}
+
+// keep + (send of the lane the DPP control selects): one step of a
+// reduce-scatter in which paired lanes exchange the halves they give up.
+template <int kCtrl>
+__device__ __forceinline__ float dpp_pair_add(float keep, float send) {
+    return keep + __int_as_float(
+                      __builtin_amdgcn_update_dpp(0, __float_as_int(send), kCtrl, 0xf, 0xf, false));
+}
+
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int lane = lane_id();
 #pragma unroll
@@ -202,6 +220,7 @@ __device__ __forceinline__ int shared_levels(unsigned long long a, unsigned long
 __device__ __forceinline__ void reset_counters(Counters* c) {
     c->pp = 0;
     c->pc = 0;
+    c->pf = 0;
     c->max_frontier = 0;
     c->overflow = 0;
     c->oversize = 0;
@@ -1242,7 +1261,21 @@ __global__ __launch_bounds__(kBandThreads) void k_upsweep_band(
 // atomic counter.  The breadth-first frontier holds the child ranges of
 // opened cells and lives in a per-block slice of global scratch (it can reach
 // thousands of entries in dense regions); interaction lists live in LDS.
-template <bool kQuad, bool kEps0>
+//
+// Far cells (kFar): an accepted cell whose centre of mass is far from the
+// group's box centre c compared with the box's half diagonal h
+// (h^2 < rho2 |R|^2, R = com - c) is not evaluated at every target.  Its field
+// enters a local Taylor expansion of the acceleration about c, to third order:
+//   a(c + d) = L1 - L2.d + 1/2 L3:dd - 1/6 L4:ddd,
+// with L1 the softened monopole + quadrupole acceleration at c, L2 the
+// monopole + quadrupole field gradient and L3, L4 the monopole's higher
+// derivatives (-M d^n(1/r)/dR^n, traceless, unsoftened).  L2..L4 are stored
+// scaled by h^(n-1) against d/h, so every stored number is bounded by the
+// cell's M/r^2.  Far cells are staged by node id; each lane computes one
+// cell's 24 coefficients (3 + 5 + 7 + 9 independent components), the wave sums
+// them (DPP, fixed order) and lane 63 adds the sums to the group's LDS
+// accumulators.  Each target evaluates the expansion once at the end.
+template <bool kQuad, bool kEps0, bool kFar>
 __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict__ spos,
                                              const unsigned* __restrict__ perm,
                                              const float4* __restrict__ mac,
@@ -1255,7 +1288,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                                              const int* __restrict__ gstart,
                                              const int* __restrict__ gorder,
                                              const LevelInfo* __restrict__ li,
-                                             Counters* __restrict__ ctr) {
+                                             Counters* __restrict__ ctr, float rho2) {
     // A tree that failed the input, sort, build or up-sweep checks is not
     // walked; this evaluation's accelerations are NaN rather than stale.
     if (ctr->overflow & kTreeBad) {
@@ -1270,6 +1303,8 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
     __shared__ float4 pl[128];                 // staged source particles
     __shared__ float4 sc0[64], sc1[64], sc2[64];  // staged cells: centre of mass, (M Qxx Qyy Qzz), (Qxy Qxz Qyz)
     __shared__ int2 fl[2][64];                 // frontier entries 0..63 of this level / the next
+    __shared__ int flist[kFar ? 64 : 1];       // staged far cells (node ids)
+    __shared__ float lexp[kFar ? 24 : 1];      // the group's local expansion coefficients
     __shared__ int sgroup;
 
     const int lane = threadIdx.x;
@@ -1278,7 +1313,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
     int2* fr0 = scratch + (size_t)blockIdx.x * 2 * fcap;
     int2* fr1 = fr0 + fcap;
     const int ngroups = li->n_groups;
-    unsigned long long tot_pp = 0, tot_pc = 0;
+    unsigned long long tot_pp = 0, tot_pc = 0, tot_pf = 0;
     int maxf = 0;
     bool ovf = false;
 
@@ -1325,6 +1360,15 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
         const float bc0 = 0.5f * (lo0 + hi0), bc1 = 0.5f * (lo1 + hi1), bc2 = 0.5f * (lo2 + hi2);
         const float bh0 = fmaxf(hi0 - bc0, bc0 - lo0), bh1 = fmaxf(hi1 - bc1, bc1 - lo1),
                     bh2 = fmaxf(hi2 - bc2, bc2 - lo2);
+        // Far-cell test: h^2 < rho2 |R|^2 with h the box's half diagonal.
+        const float rg2 = fmaf(bh0, bh0, fmaf(bh1, bh1, bh2 * bh2));
+        const float hg = sqrtf(rg2), hinv = hg > 0.f ? 1.f / hg : 0.f;
+        int nfar = 0;
+        unsigned long long npf = 0;
+        if (kFar) {
+            __syncthreads();  // the previous group's expansion has been read
+            if (lane < 24) lexp[lane] = 0.f;
+        }
 
         float fxA = 0.f, fyA = 0.f, fzA = 0.f, fxB = 0.f, fyB = 0.f, fzB = 0.f;
         int ncl = 0, npl = 0, nnext = 0, nb = 0;
@@ -1425,6 +1469,124 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             __syncthreads();
         };
 
+        // Adds the local expansion terms of the cnt (<= 64) staged far cells,
+        // one per lane, to lexp (a lane past cnt evaluates a massless cell at
+        // unit distance, which adds exact zeros); the staging slots are free
+        // afterwards.
+        auto flush_far = [&](int cnt) {
+            __syncthreads();
+            const bool valid = lane < cnt;
+            const int fid = valid ? flist[lane] : 0;
+            __syncthreads();
+            const unsigned o16 = (unsigned)fid << 4;
+            const float4 mc = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(mac) + o16);
+            float4 qa = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(q0) + o16);
+            float4 qb = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(q1) + o16);
+            float x = mc.x - bc0, y = mc.y - bc1, z = mc.z - bc2;
+            if (!valid) {
+                x = 1.f;
+                y = 0.f;
+                z = 0.f;
+                qa = make_float4(0.f, 0.f, 0.f, 0.f);
+                qb = qa;
+            }
+            if (!kQuad) qb = make_float4(0.f, 0.f, 0.f, 0.f);
+            float c[24];
+            const float M = qa.x;
+            const float Qxx = kQuad ? qa.y : 0.f, Qyy = kQuad ? qa.z : 0.f, Qzz = kQuad ? qa.w : 0.f;
+            // Order 0: softened monopole + quadrupole acceleration at the box centre.
+            const float r2s = fmaf(x, x, fmaf(y, y, fmaf(z, z, soft2)));
+            const float is = __builtin_amdgcn_rsqf(r2s);
+            const float is2 = is * is;
+            const float qdx = fmaf(Qxx, x, fmaf(qb.x, y, qb.y * z));
+            const float qdy = fmaf(qb.x, x, fmaf(Qyy, y, qb.z * z));
+            const float qdz = fmaf(qb.y, x, fmaf(qb.z, y, Qzz * z));
+            const float dqd = fmaf(x, qdx, fmaf(y, qdy, z * qdz));
+            const float is5 = (is2 * is2) * is;
+            const float sc = fmaf(2.5f, dqd * is2, M * r2s);
+            c[0] = fmaf(x, sc, -qdx) * is5;
+            c[1] = fmaf(y, sc, -qdy) * is5;
+            c[2] = fmaf(z, sc, -qdz) * is5;
+            // Orders 1-3, unsoftened, in the unit vector n = R/r: order n is
+            // stored times h^(n-1).  g2 = 1/r^2, hw = h/r.
+            const float r2 = fmaf(x, x, fmaf(y, y, z * z));
+            const float w = __builtin_amdgcn_rsqf(r2);
+            const float g2 = w * w;
+            const float nx = x * w, ny = y * w, nz = z * w;
+            const float hw = hg * w;
+            const float k2 = hw * g2;      // h/r^3
+            const float k3 = M * hw * k2;  // M h^2/r^4
+            const float k4 = k3 * hw;      // M h^3/r^5
+            // L2 h = (h/r^3) [(M + 5/2 s) I - (3M + 35/2 s) nn + 5 (n p + p n) - Q/r^2],
+            // p = Qn/r^2, s = n.p (traceless: xx, yy, xy, xz, yz).
+            const float sig = dqd * (g2 * g2);
+            const float wg = w * g2;
+            const float px = qdx * wg, py = qdy * wg, pz = qdz * wg;
+            const float A = -fmaf(17.5f, sig, 3.f * M) * k2;
+            const float C = fmaf(2.5f, sig, M) * k2;
+            const float B = 5.f * k2;
+            const float D = -k2 * g2;
+            c[3] = fmaf(A, nx * nx, fmaf(2.f * B, nx * px, fmaf(D, Qxx, C)));
+            c[4] = fmaf(A, ny * ny, fmaf(2.f * B, ny * py, fmaf(D, Qyy, C)));
+            c[5] = fmaf(A, nx * ny, fmaf(B, fmaf(nx, py, ny * px), D * qb.x));
+            c[6] = fmaf(A, nx * nz, fmaf(B, fmaf(nx, pz, nz * px), D * qb.y));
+            c[7] = fmaf(A, ny * nz, fmaf(B, fmaf(ny, pz, nz * py), D * qb.z));
+            // L3 h^2 = (M h^2/r^4) (15 nnn - 3 sym(n I)): xxx, xxy, xxz, xyy, xyz, yyy, yyz.
+            const float e = 15.f * k3, f = 3.f * k3;
+            const float xx = nx * nx, yy = ny * ny;
+            const float ex = fmaf(e, xx, -f), ey = fmaf(e, yy, -f);
+            c[8] = nx * fmaf(-2.f, f, ex);
+            c[9] = ny * ex;
+            c[10] = nz * ex;
+            c[11] = nx * ey;
+            c[12] = e * (nx * ny) * nz;
+            c[13] = ny * fmaf(-2.f, f, ey);
+            c[14] = nz * ey;
+            // L4 h^3 = (M h^3/r^5) (-105 nnnn + 15 sym(nn I) - 3 sym(I I)):
+            // xxxx, xxxy, xxxz, xxyy, xxyz, xyyy, xyyz, yyyy, yyyz.
+            const float a105 = -105.f * k4;
+            const float ux = fmaf(a105, xx, 45.f * k4), uy = fmaf(a105, yy, 45.f * k4);
+            const float vx = fmaf(a105, xx, 15.f * k4), vy = fmaf(a105, yy, 15.f * k4);
+            c[15] = fmaf(xx, fmaf(a105, xx, 90.f * k4), -9.f * k4);
+            c[16] = (nx * ny) * ux;
+            c[17] = (nx * nz) * ux;
+            c[18] = fmaf(xx, fmaf(a105, yy, 15.f * k4), fmaf(15.f * k4, yy, -3.f * k4));
+            c[19] = (ny * nz) * vx;
+            c[20] = (nx * ny) * uy;
+            c[21] = (nx * nz) * vy;
+            c[22] = fmaf(yy, fmaf(a105, yy, 90.f * k4), -9.f * k4);
+            c[23] = (ny * nz) * uy;
+            // Reduce-scatter over the wave: three pair exchanges inside each
+            // 16-lane row (row_mirror, row_half_mirror, quad_perm 3210) halve
+            // the coefficients a lane holds, 24 -> 12 -> 6 -> 3; the pair i, i^1
+            // and the four rows are then summed.  Lane bits 3, 2, 1 select
+            // coefficients 12 b3 + 6 b2 + 3 b1 + (0, 1, 2).  Fixed order: the
+            // sums are deterministic.
+            const bool u8 = lane & 8, u4 = lane & 4, u2 = lane & 2;
+            float h12[12], h6[6], h3[3];
+#pragma unroll
+            for (int k = 0; k < 12; ++k)
+                h12[k] = dpp_pair_add<0x140>(u8 ? c[k + 12] : c[k], u8 ? c[k] : c[k + 12]);
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                h6[k] = dpp_pair_add<0x141>(u4 ? h12[k + 6] : h12[k], u4 ? h12[k] : h12[k + 6]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                h3[k] = dpp_pair_add<0x1B>(u2 ? h6[k + 3] : h6[k], u2 ? h6[k] : h6[k + 3]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                h3[k] = dpp_add<0xB1>(h3[k]);
+                h3[k] += __shfl_xor(h3[k], 16, 64);
+                h3[k] += __shfl_xor(h3[k], 32, 64);
+            }
+            if (lane < 16 && !(lane & 1)) {
+                const int base = (u8 ? 12 : 0) + (u4 ? 6 : 0) + (u2 ? 3 : 0);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) lexp[base + k] += h3[k];
+            }
+            npf += (unsigned long long)cnt;
+        };
+
         // Classifies one candidate node per lane (id < 0: none) against the
         // group.  The node's walk record is read once: an accepted cell goes
         // from registers to the LDS staging list.
@@ -1445,6 +1607,12 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             const bool acc = valid && d2 > mc.w;
             const bool leaf = valid && !acc && lk.x < 0;
             const bool open = valid && !acc && lk.x >= 0;
+            bool far = false;
+            if (kFar && acc) {
+                const float rx = mc.x - bc0, ry = mc.y - bc1, rz = mc.z - bc2;
+                far = rg2 < rho2 * fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+            }
+            const bool nacc = acc && !far;  // accepted, evaluated at every target
 
             const unsigned long long bo = __ballot(open);
             const int no = __popcll(bo);
@@ -1459,10 +1627,10 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             // Accepted cells: staged in LDS and evaluated cfull at a time
             // (cfull is a multiple of the subset count, so no subset idles).
             // Cells past a full batch wait in registers during its evaluation.
-            const unsigned long long ba = __ballot(acc);
+            const unsigned long long ba = __ballot(nacc);
             const int na = __popcll(ba);
             const int pos = ncl + __popcll(ba & lt);  // < cfull + 63
-            if (acc && pos < cfull) {
+            if (nacc && pos < cfull) {
                 sc0[pos] = mc;
                 sc1[pos] = qa;
                 if (kQuad) sc2[pos] = qb;
@@ -1470,7 +1638,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             ncl += na;
             if (ncl >= cfull) {
                 flush_cells(cfull);
-                if (acc && pos >= cfull) {
+                if (nacc && pos >= cfull) {
                     sc0[pos - cfull] = mc;
                     sc1[pos - cfull] = qa;
                     if (kQuad) sc2[pos - cfull] = qb;
@@ -1492,6 +1660,22 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                         if (kQuad) sc2[lane] = v2;
                     }
                     ncl -= cfull;
+                }
+            }
+
+            // Far cells: node ids staged 64 at a time; ids past a full batch
+            // wait in registers during its evaluation.
+            if (kFar) {
+                const unsigned long long bf = __ballot(far);
+                if (bf) {
+                    const int fpos = nfar + __popcll(bf & lt);  // < 64 + 63
+                    if (far && fpos < 64) flist[fpos] = idc;
+                    nfar += __popcll(bf);
+                    if (nfar >= 64) {
+                        flush_far(64);
+                        if (far && fpos >= 64) flist[fpos - 64] = idc;
+                        nfar -= 64;
+                    }
                 }
             }
 
@@ -1574,6 +1758,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
         }
         if (ncl > 0) flush_cells(ncl);
         if (npl > 0) flush_particles(npl);
+        if (kFar && nfar > 0) flush_far(nfar);
 
         if (nsub > 1) {  // reduce the subsets' partial sums in subset order
             sc0[lane] = make_float4(fxA, fyA, fzA, 0.f);
@@ -1593,6 +1778,54 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             }
             __syncthreads();
         }
+        if (kFar && npf > 0) {  // the far cells' expansion at each target
+            __syncthreads();      // lane 63's sums are visible
+            float L[24];
+#pragma unroll
+            for (int j = 0; j < 24; ++j) L[j] = lexp[j];
+            // Order 1 (zz = -xx - yy), order 2 (xzz, yzz, zzz from the trace
+            // conditions), order 3 (the six components with z^2 likewise).
+            const float t200 = L[3], t020 = L[4], t110 = L[5], t101 = L[6], t011 = L[7];
+            const float t002 = -t200 - t020;
+            const float t300 = L[8], t210 = L[9], t201 = L[10], t120 = L[11], t111 = L[12],
+                        t030 = L[13], t021 = L[14];
+            const float t102 = -t300 - t120, t012 = -t210 - t030, t003 = -t201 - t021;
+            const float u400 = L[15], u310 = L[16], u301 = L[17], u220 = L[18], u211 = L[19],
+                        u130 = L[20], u121 = L[21], u040 = L[22], u031 = L[23];
+            const float u202 = -u400 - u220, u112 = -u310 - u130, u022 = -u220 - u040,
+                        u103 = -u301 - u121, u013 = -u211 - u031, u004 = -u202 - u022;
+            auto l2p = [&](const float4& me, float& fx, float& fy, float& fz) {
+                const float p = (me.x - bc0) * hinv, q = (me.y - bc1) * hinv, s = (me.z - bc2) * hinv;
+                const float g2x = fmaf(t200, p, fmaf(t110, q, t101 * s));
+                const float g2y = fmaf(t110, p, fmaf(t020, q, t011 * s));
+                const float g2z = fmaf(t101, p, fmaf(t011, q, t002 * s));
+                const float pp = p * p, qq = q * q, ss = s * s, pq = p * q, ps = p * s, qs = q * s;
+                const float g3x = fmaf(t300, pp, fmaf(t120, qq, fmaf(t102, ss,
+                                  2.f * fmaf(t210, pq, fmaf(t201, ps, t111 * qs)))));
+                const float g3y = fmaf(t210, pp, fmaf(t030, qq, fmaf(t012, ss,
+                                  2.f * fmaf(t120, pq, fmaf(t111, ps, t021 * qs)))));
+                const float g3z = fmaf(t201, pp, fmaf(t021, qq, fmaf(t003, ss,
+                                  2.f * fmaf(t111, pq, fmaf(t102, ps, t012 * qs)))));
+                const float p3 = pp * p, q3 = qq * q, s3 = ss * s, pqs = pq * s;
+                const float g4x = fmaf(u400, p3, fmaf(u130, q3, fmaf(u103, s3, fmaf(6.f * u211, pqs,
+                                  3.f * fmaf(u310, pp * q, fmaf(u301, pp * s, fmaf(u220, p * qq,
+                                  fmaf(u121, qq * s, fmaf(u202, p * ss, u112 * (q * ss))))))))));
+                const float g4y = fmaf(u310, p3, fmaf(u040, q3, fmaf(u013, s3, fmaf(6.f * u121, pqs,
+                                  3.f * fmaf(u220, pp * q, fmaf(u211, pp * s, fmaf(u130, p * qq,
+                                  fmaf(u031, qq * s, fmaf(u112, p * ss, u022 * (q * ss))))))))));
+                const float g4z = fmaf(u301, p3, fmaf(u031, q3, fmaf(u004, s3, fmaf(6.f * u112, pqs,
+                                  3.f * fmaf(u211, pp * q, fmaf(u202, pp * s, fmaf(u121, p * qq,
+                                  fmaf(u022, qq * s, fmaf(u103, p * ss, u013 * (q * ss))))))))));
+                constexpr float kSixth = 1.f / 6.f;
+                fx += fmaf(-kSixth, g4x, fmaf(0.5f, g3x, L[0] - g2x));
+                fy += fmaf(-kSixth, g4y, fmaf(0.5f, g3y, L[1] - g2y));
+                fz += fmaf(-kSixth, g4z, fmaf(0.5f, g3z, L[2] - g2z));
+            };
+            if (lead) {
+                l2p(meA, fxA, fyA, fzA);
+                if (hasB) l2p(meB, fxB, fyB, fzB);
+            }
+        }
         // A group whose frontier overflowed skipped cells: its accelerations
         // are NaN (error bit 1), never silently incomplete.
         ovf = ovf || gov;
@@ -1611,11 +1844,13 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
         }
         tot_pp += npp * (unsigned long long)nact;
         tot_pc += npc * (unsigned long long)nact;
+        tot_pf += npf;
       }
     }
     if (lane == 0) {
         atomicAdd(&ctr->pp, tot_pp);
         atomicAdd(&ctr->pc, tot_pc);
+        if (kFar) atomicAdd(&ctr->pf, tot_pf);
         atomicMax(&ctr->max_frontier, maxf);
         if (ovf) raise_flag(ctr, 1);
     }
@@ -1675,6 +1910,16 @@ int walk_fcap(double theta) {
     if (!(theta < 0.5)) return kWalkFcap;
     const double r = 0.5 / std::max(theta, 1e-3);
     return (int)std::min<double>(kWalkFcapMax, std::ceil(kWalkFcap * r * r * r / 256.0) * 256.0);
+}
+
+// Far-cell ratio of the quadrupole walk: a cell accepted for a group goes
+// into the group's local expansion when the group box's half diagonal is
+// below ratio * theta times the distance from the box centre to the cell's
+// centre of mass.  NB_WALK_FAR (tuning) sets the ratio; 0 evaluates every
+// accepted cell at every target.
+double walk_far_ratio() {
+    if (const char* e = std::getenv("NB_WALK_FAR")) return std::max(0.0, std::min(1.0, std::atof(e)));
+    return kWalkFarRatio;
 }
 
 int64_t walk_scratch_bytes(double theta, int cus) {
@@ -1954,18 +2199,26 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_TRY(w->walk_scratch.reserve((size_t)w->walk_blocks * 2 * fcap * sizeof(int2)));
     const float s2 = (float)soft2;
     const bool quad = multipole == NB_QUADRUPOLE;
-#define NB_WALK(Q, E)                                                                          \
-    hipLaunchKernelGGL((k_walk<Q, E>), dim3(wblocks), dim3(64), 0, s, n, w->spos.as<float4>(), \
-                       v0, w->w_mac.as<float4>(), w->w_link.as<int2>(), w->w_q0.as<float4>(),  \
-                       w->w_q1.as<float4>(), s2, G, ax, ay, az,                                 \
+#define NB_WALK(Q, E, F)                                                                       \
+    hipLaunchKernelGGL((k_walk<Q, E, F>), dim3(wblocks), dim3(64), 0, s, n,                      \
+                       w->spos.as<float4>(), v0, w->w_mac.as<float4>(), w->w_link.as<int2>(),   \
+                       w->w_q0.as<float4>(), w->w_q1.as<float4>(), s2, G, ax, ay, az,          \
                        w->walk_scratch.as<int2>(), fcap, w->gstart.as<int>(),                \
-                       w->gorder.as<int>(), li, ctr)
+                       w->gorder.as<int>(), li, ctr, rho2)
+    const double rho = walk_far_ratio() * theta;
+    const float rho2 = (float)(rho * rho);
+    const bool far = quad && rho2 > 0.f;
     if (quad) {
-        if (s2 > 0.f) NB_WALK(true, false);
-        else NB_WALK(true, true);
+        if (s2 > 0.f) {
+            if (far) NB_WALK(true, false, true);
+            else NB_WALK(true, false, false);
+        } else {
+            if (far) NB_WALK(true, true, true);
+            else NB_WALK(true, true, false);
+        }
     } else {
-        if (s2 > 0.f) NB_WALK(false, false);
-        else NB_WALK(false, true);
+        if (s2 > 0.f) NB_WALK(false, false, false);
+        else NB_WALK(false, true, false);
     }
 #undef NB_WALK
     NB_LAUNCH_CHECK();
@@ -1995,6 +2248,7 @@ int bh_stats(BHWorkspace* w, hipStream_t s, nb_bh_stats* out) {
     out->oversize_cells = c.oversize;
     out->pp_interactions = (int64_t)c.pp;
     out->pc_interactions = (int64_t)c.pc;
+    out->far_cells = (int64_t)c.pf;
     out->max_frontier = c.max_frontier;
     out->frontier_overflow = c.overflow;
     for (int a = 0; a < 3; ++a) out->root_center[a] = r.center[a];
