@@ -116,21 +116,26 @@ def rel_err(a, r):
 
 
 # --------------------------------------------------------------------- FP32 ceiling
-# Instructions of the compiled interaction terms besides their one v_rsq_f32
-# (gfx950 ISA of the inner loops): the direct FP32 term with softening and the
-# walk's particle term are 3 sub + 3 mul + 6 FMA (20 FLOP); the walk's
-# quadrupole cell term is 3 add/sub + 9 mul + 18 FMA (60 FLOP).
-TERM_OTHER_INSTR = {"pp": 12, "pc": 30}
-TERM_FLOP = {"pp": 20, "pc": 60}
+# Instructions of the compiled terms besides their v_rsq_f32 (gfx950 ISA):
+# the direct FP32 term with softening and the walk's particle term are 3 sub +
+# 3 mul + 6 FMA (20 FLOP); the walk's quadrupole cell term is 3 add/sub + 9 mul
+# + 18 FMA (60 FLOP).  A far cell (one lane builds its 24 local-expansion
+# coefficients: 117 VALU + 2 rsqrt, 164 FLOP; the wave's reduce-scatter of a
+# batch of 64 adds 82 VALU per lane, 30 of them adds) costs 199 other
+# instructions and 194 FLOP; evaluating the expansion at a target, 105 VALU
+# and 160 FLOP.
+TERM_OTHER_INSTR = {"pp": 12, "pc": 30, "far": 199, "l2p": 105}
+TERM_RSQ = {"pp": 1, "pc": 1, "far": 2, "l2p": 0}
+TERM_FLOP = {"pp": 20, "pc": 60, "far": 194, "l2p": 160}
 
 
 def fma_probe(nb, achieved, terms=None):
     """The FP32 ceiling without packed FP32 (the library is built without it):
     v_fma_f32 issue rate with VGPR operands, measured in this run outside the
     timed region, and the kernel's FLOP rate as a fraction of it.  With
-    `terms` ({"pp": count, "pc": count}), also the ceiling of that term mix:
-    the FMA slot rate (probe / 2 FLOP) times FLOP per slot, a term taking its
-    other instructions plus the measured rsqrt cost in slots."""
+    `terms` ({"pp": count, "pc": count, ...}), also the ceiling of that term
+    mix: the FMA slot rate (probe / 2 FLOP) times FLOP per slot, a term taking
+    its other instructions plus its rsqrts at the measured cost in slots."""
     r = nb.probe_fp32_fma()
     out = {"tflops": round(r["tflops"], 2), "frac": round(achieved / r["tflops"], 4),
            "what": "16 independent v_fma_f32 chains per lane, VGPR operands, 8 x 256-lane "
@@ -138,12 +143,23 @@ def fma_probe(nb, achieved, terms=None):
     if terms:
         c = nb.probe_fp32_rsq_cost()
         flop = sum(TERM_FLOP[k] * v for k, v in terms.items())
-        slots = sum((TERM_OTHER_INSTR[k] + c) * v for k, v in terms.items())
+        slots = sum((TERM_OTHER_INSTR[k] + TERM_RSQ[k] * c) * v for k, v in terms.items())
         ceiling = r["tflops"] / 2.0 * flop / slots
         out["rsq_fma_slots"] = round(c, 2)
         out["mix_ceiling_tflops"] = round(ceiling, 2)
         out["mix_frac"] = round(achieved / ceiling, 4)
     return out
+
+
+def walk_terms(st, n):
+    """Term counts of one walk launch: particle and cell terms per target, far
+    cells (one local-expansion build each) and, when there are far cells, one
+    expansion evaluation per target."""
+    t = {"pp": st["pp_interactions"], "pc": st["pc_interactions"]}
+    if st.get("far_cells", 0) > 0:
+        t["far"] = st["far_cells"]
+        t["l2p"] = n
+    return t
 
 
 # --------------------------------------------------------------------- Barnes-Hut (headline)
@@ -283,8 +299,13 @@ def bench_bh(args):
 
     # Walk roofline (FP32 VALU): interactions of the last evaluation are
     # counted by the kernel; the tree barely changes over the timed steps.
-    flops = FLOP_PP * st["pp_interactions"] + FLOP_PC_QUAD * st["pc_interactions"]
+    terms = walk_terms(st, n)
+    flops = sum(TERM_FLOP[k] * v for k, v in terms.items())
     achieved = flops / (walk_ms * 1e-3) / 1e12
+    # The same accepted cells evaluated at every target (NB_WALK_FAR=0) would
+    # cost 60 FLOP per (target, cell) pair: the rate that work is done at.
+    equiv = (FLOP_PP * st["pp_interactions"]
+             + FLOP_PC_QUAD * (st["pc_interactions"] + st["far_target_terms"]))
     roofline = {"kernel": "k_walk", "bound": "valu", "achieved": round(achieved, 2),
                 "peak_kind": "FP32 vector (157.3 TFLOP/s is also the dense FP32 MFMA figure)",
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
@@ -292,9 +313,15 @@ def bench_bh(args):
                 "launch_ms": round(walk_ms, 4),
                 "flops_per_launch": int(flops),
                 "interactions_per_particle": {"pp": round(st["pp_interactions"] / n, 1),
-                                              "pc": round(st["pc_interactions"] / n, 1)},
-                "fma_probe": fma_probe(nb, achieved, {"pp": st["pp_interactions"],
-                                                      "pc": st["pc_interactions"]})}
+                                              "pc": round(st["pc_interactions"] / n, 1),
+                                              "pc_in_far_expansions": round(st["far_target_terms"] / n, 1)},
+                "far_cells_per_launch": st["far_cells"],
+                "per_target_equivalent": {
+                    "flops_per_launch": int(equiv),
+                    "tflops": round(equiv / (walk_ms * 1e-3) / 1e12, 2),
+                    "what": "FLOP of evaluating every accepted cell at every target (60 per pair) "
+                            "and the particle terms (20), over the walk time"},
+                "fma_probe": fma_probe(nb, achieved, terms)}
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": elapsed / K * 1e3, "higher_is_better": True,

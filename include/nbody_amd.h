@@ -199,6 +199,9 @@ typedef struct nb_bh_stats {
     int32_t leaf_capacity;     /* leaf capacity of the last evaluation */
     int64_t far_cells;         /* (walk group, cell) pairs whose field entered the group's local
                                   expansion instead of the per-target sum (quadrupole walk) */
+    int64_t far_target_terms;  /* (target, cell) pairs those expansions stand for:
+                                  pc_interactions + far_target_terms is the number of accepted
+                                  (target, cell) pairs of the walk */
 } nb_bh_stats;
 /* Fills *out and returns NB_OK, or (with *out filled) NB_ERR_CAPACITY if an
  * evaluation since the previous call set flag 1, 2, 4 or 8 (its NaN

@@ -28,7 +28,7 @@
 //             mass, centre of mass and the traceless quadrupole in FP64 for
 //             every level; FP32 copies for the walk.
 //   walk      k_walk: persistent one-wave blocks take walk groups (the tree
-//             cells of at most 64 particles) from an atomic counter, in
+//             cells of at most 128 particles) from an atomic counter, in
 //             size classes largest first, key order inside a class (the order
 //             is written by k_emit).  The wave
 //             opens cells breadth-first (8 parents x 8 children per step, one
@@ -42,7 +42,10 @@
 //             second one, and each list is evaluated by all lanes (two targets
 //             per lane) in batches of 60-64 entries, a multiple of the number
 //             of lane subsets.  The first 64 frontier entries of a level are
-//             kept in LDS.
+//             kept in LDS.  An accepted cell far from the group compared with
+//             the group's size (quadrupole walk) instead enters a third-order
+//             local expansion about the group centre, evaluated once per
+//             target.
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -88,13 +91,15 @@ constexpr int kLookbackSpins = 1 << 22;  // polls before a tile gives up waiting
 constexpr int kWalkFcap = 2048;
 constexpr int kWalkFcapMax = 65536;
 constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
-constexpr int kGroup = 64;      // walk group: cells of at most 64 particles
-constexpr double kWalkFarRatio = 0.3;  // far cells: box half diagonal < 0.3 theta |R|
+constexpr int kGroup = 128;     // largest walk group (a wave holds two targets per lane)
+constexpr int kGroupDefault = 128;  // walk group: cells of at most 128 particles
+constexpr double kWalkFarRatio = 0.4;  // far cells: box half diagonal < 0.4 theta |R|
 constexpr int64_t kMaxNodes = (int64_t)1 << 28;  // node pool limit (32-bit record offsets in the walk)
 // Per-block count columns: 22 levels, group starts, leaf starts, oversize leaves.
 // Walk groups are handed out by size class, largest first (a group's cost
 // grows with its size: one interaction list evaluated for all its targets),
-// in key order inside a class: classes >= 48, 32-47, 16-31 and < 16 particles.
+// in key order inside a class: with groups of at most g particles, classes
+// >= 3g/4, g/2 .. 3g/4-1, g/4 .. g/2-1 and < g/4 particles.
 constexpr int kGBuckets = 4;
 constexpr int kColGroup = kLevels, kColLeaf = kLevels + 1, kColOver = kLevels + 2;
 constexpr int kColGB = kLevels + 3;  // kGBuckets columns: group starts per size class
@@ -119,7 +124,8 @@ struct LevelInfo {
 
 struct Counters {
     unsigned long long pp, pc;
-    unsigned long long pf;  // far cells: (group, cell) pairs taken into a group's local expansion
+    unsigned long long pf;   // far cells: (group, cell) pairs taken into a group's local expansion
+    unsigned long long pft;  // (target, far cell) pairs those expansions stand for
     int max_frontier;
     int overflow;   // this evaluation: bit 0 walk frontier, bit 1 node capacity, bit 2
                     // malformed tree, bit 3 sort look-back wait limit, bit 4 non-finite input
@@ -221,6 +227,7 @@ __device__ __forceinline__ void reset_counters(Counters* c) {
     c->pp = 0;
     c->pc = 0;
     c->pf = 0;
+    c->pft = 0;
     c->max_frontier = 0;
     c->overflow = 0;
     c->oversize = 0;
@@ -708,8 +715,8 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
 // halos for large n).
 //
 // A walk group is the tree cell of the smallest level that holds at most
-// kGroup particles (or the particle's leaf if that is shallower), so a group
-// is spatially compact; its particles are contiguous in sorted order.
+// gsz (<= kGroup) particles (or the particle's leaf if that is shallower), so
+// a group is spatially compact; its particles are contiguous in sorted order.
 template <int kRows>
 __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long* __restrict__ keys,
                                                  int cap, int gsz, const RootInfo* __restrict__ root,
@@ -802,7 +809,10 @@ __global__ __launch_bounds__(kBlk) void k_levels(int n, const unsigned long long
             auto longer = [&](int k) {
                 return i + k < n && shared_levels(sk[H + o], sk[H + o + k]) >= gl;
             };
-            gb = longer(47) ? 0 : longer(31) ? 1 : longer(15) ? 2 : 3;
+            gb = longer(max(0, 3 * gsz / 4 - 1)) ? 0
+                 : longer(max(0, gsz / 2 - 1)) ? 1
+                 : longer(max(0, gsz / 4 - 1)) ? 2
+                                                : 3;
         }
         if (valid) gflag[i] = (unsigned char)(gb + 1);  // 0: no group starts here
         const bool over = leaf_start && mx + 1 > lmax && key_limited;
@@ -1271,10 +1281,11 @@ __global__ __launch_bounds__(kBandThreads) void k_upsweep_band(
 // monopole + quadrupole field gradient and L3, L4 the monopole's higher
 // derivatives (-M d^n(1/r)/dR^n, traceless, unsoftened).  L2..L4 are stored
 // scaled by h^(n-1) against d/h, so every stored number is bounded by the
-// cell's M/r^2.  Far cells are staged by node id; each lane computes one
-// cell's 24 coefficients (3 + 5 + 7 + 9 independent components), the wave sums
-// them (DPP, fixed order) and lane 63 adds the sums to the group's LDS
-// accumulators.  Each target evaluates the expansion once at the end.
+// cell's M/r^2.  Far cells are staged by node id, 64 at a time; each lane
+// computes one cell's 24 coefficients (3 + 5 + 7 + 9 independent components),
+// a fixed-order reduce-scatter sums them over the wave and eight lanes add the
+// sums to the group's LDS accumulators.  Each target evaluates the expansion
+// once at the end.
 template <bool kQuad, bool kEps0, bool kFar>
 __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict__ spos,
                                              const unsigned* __restrict__ perm,
@@ -1313,7 +1324,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
     int2* fr0 = scratch + (size_t)blockIdx.x * 2 * fcap;
     int2* fr1 = fr0 + fcap;
     const int ngroups = li->n_groups;
-    unsigned long long tot_pp = 0, tot_pc = 0, tot_pf = 0;
+    unsigned long long tot_pp = 0, tot_pc = 0, tot_pf = 0, tot_pft = 0;
     int maxf = 0;
     bool ovf = false;
 
@@ -1845,12 +1856,16 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
         tot_pp += npp * (unsigned long long)nact;
         tot_pc += npc * (unsigned long long)nact;
         tot_pf += npf;
+        tot_pft += npf * (unsigned long long)nact;
       }
     }
     if (lane == 0) {
         atomicAdd(&ctr->pp, tot_pp);
         atomicAdd(&ctr->pc, tot_pc);
-        if (kFar) atomicAdd(&ctr->pf, tot_pf);
+        if (kFar) {
+            atomicAdd(&ctr->pf, tot_pf);
+            atomicAdd(&ctr->pft, tot_pft);
+        }
         atomicMax(&ctr->max_frontier, maxf);
         if (ovf) raise_flag(ctr, 1);
     }
@@ -1914,9 +1929,10 @@ int walk_fcap(double theta) {
 
 // Far-cell ratio of the quadrupole walk: a cell accepted for a group goes
 // into the group's local expansion when the group box's half diagonal is
-// below ratio * theta times the distance from the box centre to the cell's
-// centre of mass.  NB_WALK_FAR (tuning) sets the ratio; 0 evaluates every
-// accepted cell at every target.
+// below ratio * min(theta, 0.5) times the distance from the box centre to the
+// cell's centre of mass (the expansion's fourth-order remainder then stays
+// below the quadrupole's own truncation error).  NB_WALK_FAR (tuning) sets
+// the ratio; 0 evaluates every accepted cell at every target.
 double walk_far_ratio() {
     if (const char* e = std::getenv("NB_WALK_FAR")) return std::max(0.0, std::min(1.0, std::atof(e)));
     return kWalkFarRatio;
@@ -2075,7 +2091,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     // (The gather ran in the last radix pass: the gather stage is empty.)
     if (ev && ev[4]) NB_HIP(hipEventRecord(ev[4], s));
     if (w->group_size == 0) {  // walk group size: NB_WALK_GROUP overrides the default
-        w->group_size = kGroup;
+        w->group_size = kGroupDefault;
         if (const char* e = std::getenv("NB_WALK_GROUP"))
             w->group_size = std::max(1, std::min(kGroup, std::atoi(e)));
     }
@@ -2205,7 +2221,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->w_q0.as<float4>(), w->w_q1.as<float4>(), s2, G, ax, ay, az,          \
                        w->walk_scratch.as<int2>(), fcap, w->gstart.as<int>(),                \
                        w->gorder.as<int>(), li, ctr, rho2)
-    const double rho = walk_far_ratio() * theta;
+    const double rho = walk_far_ratio() * std::min(theta, 0.5);  // rho <= 0.2 by default
     const float rho2 = (float)(rho * rho);
     const bool far = quad && rho2 > 0.f;
     if (quad) {
@@ -2249,6 +2265,7 @@ int bh_stats(BHWorkspace* w, hipStream_t s, nb_bh_stats* out) {
     out->pp_interactions = (int64_t)c.pp;
     out->pc_interactions = (int64_t)c.pc;
     out->far_cells = (int64_t)c.pf;
+    out->far_target_terms = (int64_t)c.pft;
     out->max_frontier = c.max_frontier;
     out->frontier_overflow = c.overflow;
     for (int a = 0; a < 3; ++a) out->root_center[a] = r.center[a];

@@ -55,7 +55,8 @@ class _BHStats(C.Structure):
                 ("max_frontier", C.c_int32), ("frontier_overflow", C.c_int32),
                 ("root_center", C.c_double * 3), ("root_size", C.c_double),
                 ("n_groups", C.c_int64), ("flags_since_read", C.c_int32),
-                ("leaf_capacity", C.c_int32), ("far_cells", C.c_int64)]
+                ("leaf_capacity", C.c_int32), ("far_cells", C.c_int64),
+                ("far_target_terms", C.c_int64)]
 
 
 class _Diag(C.Structure):
@@ -332,7 +333,8 @@ class BarnesHutForceCalculator(ForceCalculator):
                     max_frontier=s.max_frontier, frontier_overflow=s.frontier_overflow,
                     root_center=np.array(s.root_center[:]), root_size=s.root_size,
                     n_groups=s.n_groups, flags_since_read=s.flags_since_read,
-                    leaf_capacity=s.leaf_capacity, far_cells=s.far_cells, status=rc)
+                    leaf_capacity=s.leaf_capacity, far_cells=s.far_cells,
+                    far_target_terms=s.far_target_terms, status=rc)
 
     def topology(self, n):
         depth = np.zeros(n, np.int32)

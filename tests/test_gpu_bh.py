@@ -164,9 +164,9 @@ def test_accuracy_plummer_65536(nb, mp, p99, med):
 
 @pytest.mark.parametrize("cap", [1, 33, 64])
 def test_accuracy_across_leaf_capacities(nb, cap):
-    """Leaf capacities at the ends of the accepted range [1, 64] (64 is also
-    the walk's group size, so a full leaf is a whole group): same force bar
-    as capacity 8 on the same Plummer input."""
+    """Leaf capacities at the ends of the accepted range [1, 64] (a full leaf
+    of 64 is half a walk group of 128): same force bar as capacity 8 on the
+    same Plummer input."""
     pl = nb.ic_plummer(65536, 3)
     f, p = run_bh(nb, pl, eps=0.005, theta=0.5, cap=cap, G=1.0)
     e = rel_err(p.acc(), f64_direct(nb, pl, 0.005, 1.0))
@@ -698,7 +698,7 @@ def test_plummer_16m_size_independent_properties(nb, O):
     f.computeAccelerations(p)
     st = f.stats()
     assert st["frontier_overflow"] == 0 and st["oversize_cells"] < n // 10000
-    assert st["n_nodes"] > n // 4 and st["n_groups"] > n // 64
+    assert st["n_nodes"] > n // 4 and st["n_groups"] > n // 128
     a = p.acc()
     topo = f.topology(n)
     keys, perm = topo["keys"], topo["perm"]
@@ -762,3 +762,75 @@ def test_other_distributions_sampled_accuracy(nb, O, case):
     e = rel_err(tuple(c[idx] for c in p.acc()), ref)
     assert np.percentile(e, 99) < 1e-3 and np.median(e) < 3e-4, (np.median(e), np.percentile(e, 99))
 
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
+def test_far_expansion_keeps_the_per_target_accuracy(nb, monkeypatch, theta):
+    """Accepted cells far from a walk group (the group box's half diagonal
+    below 0.4 theta times the distance to the cell's centre of mass) enter
+    the group's third-order local expansion instead of the per-target sum.
+    Against the FP64 direct sum, the error stays within 10 % of the
+    per-target walk's (NB_WALK_FAR=0, same tree and groups) at theta = 0.3,
+    0.5 and 0.7; the particle terms are unchanged and most cell terms move."""
+    pl = nb.ic_plummer(65536, 3)
+    ref = f64_direct(nb, pl, 0.005, 1.0)
+    out = {}
+    for far in ("0", "default"):
+        if far == "default":
+            monkeypatch.delenv("NB_WALK_FAR", raising=False)
+        else:
+            monkeypatch.setenv("NB_WALK_FAR", far)
+        f, p = run_bh(nb, pl, eps=0.005, theta=theta, G=1.0)
+        out[far] = (rel_err(p.acc(), ref), f.stats())
+    (e0, s0), (e1, s1) = out["0"], out["default"]
+    print(theta, np.median(e0), np.percentile(e0, 99), np.median(e1), np.percentile(e1, 99),
+          s0["pc_interactions"], s1["pc_interactions"], s1["far_cells"])
+    assert s0["far_cells"] == 0 and s1["far_cells"] > 0
+    assert s1["pp_interactions"] == s0["pp_interactions"]
+    assert s1["pc_interactions"] < 0.6 * s0["pc_interactions"]
+    assert s1["pc_interactions"] + s1["far_target_terms"] == s0["pc_interactions"]
+    assert np.percentile(e1, 99) <= 1.1 * np.percentile(e0, 99)
+    assert np.median(e1) <= 1.1 * np.median(e0)
+    assert s1["frontier_overflow"] == 0
+
+
+@pytest.mark.parametrize("eps,coincident", [(0.0, False), (0.01, False), (0.01, True)])
+def test_far_expansion_of_a_distant_cluster(nb, O, monkeypatch, eps, coincident):
+    """128 light targets in a unit ball (or all at one point: a group of zero
+    size) and 64 heavy sources 50 away: the targets feel the sources only
+    through far cells (group size / distance ~ 0.02), so the expansion's
+    error is its fourth-order remainder, ~15 rho^4 ~ 3e-6 of the force.  A
+    wrong sign or scale in the first-, second- or third-order terms would
+    leave ~3 rho, ~6 rho^2 or ~10 rho^3 >= 8e-5; the bar is 1.5e-5 against the
+    per-target walk and against the FP64 direct sum."""
+    rng = np.random.default_rng(11)
+    ta = rng.normal(size=(3, 128)) / 3.0
+    if coincident:
+        ta[:] = np.array([[0.25], [-0.5], [0.125]])
+    src = rng.normal(size=(3, 64)) * 0.3 + np.array([[50.0], [7.0], [-3.0]])
+    x, y, z = np.concatenate([ta, src], 1)
+    m = np.concatenate([np.full(128, 1e-9), np.full(64, 1.0 / 64)])
+    d = dict(m=m, x=x, y=y, z=z)
+    ref = O.direct(m, x, y, z, 1.0, eps * eps)
+    monkeypatch.setenv("NB_WALK_FAR", "0")
+    _, p0 = run_bh(nb, d, eps=eps, theta=0.5, G=1.0)
+    monkeypatch.delenv("NB_WALK_FAR")
+    f, p1 = run_bh(nb, d, eps=eps, theta=0.5, G=1.0)
+    assert f.stats()["far_cells"] > 0
+    a0, a1 = p0.acc(), p1.acc()
+    e_far = rel_err([v[:128] for v in a1], [v[:128] for v in a0])
+    e_ref = rel_err([v[:128] for v in a1], [v[:128] for v in ref])
+    assert np.isfinite(np.stack(a1)).all()
+    assert e_far.max() < 1.5e-5 and e_ref.max() < 1.5e-5, (e_far.max(), e_ref.max())
+
+
+def test_monopole_walk_has_no_far_cells(nb, monkeypatch):
+    """The far-cell expansion serves the quadrupole walk only: the monopole
+    walk (the reference's order) evaluates every accepted cell per target,
+    whatever NB_WALK_FAR says."""
+    pl = nb.ic_plummer(20000, 4)
+    f, p = run_bh(nb, pl, eps=0.005, mp="monopole", G=1.0)
+    a = np.stack(p.acc())
+    monkeypatch.setenv("NB_WALK_FAR", "0")
+    g, q = run_bh(nb, pl, eps=0.005, mp="monopole", G=1.0)
+    assert f.stats()["far_cells"] == 0 and np.array_equal(a, np.stack(q.acc()))
