@@ -91,6 +91,7 @@ constexpr int kLookbackSpins = 1 << 22;  // polls before a tile gives up waiting
 constexpr int kWalkFcap = 2048;
 constexpr int kWalkFcapMax = 65536;
 constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
+constexpr int kWalkBlocksPerCUFar = 20;  // the same for the far-cell walk (measured best)
 constexpr int kGroup = 128;     // largest walk group (a wave holds two targets per lane)
 constexpr int kGroupDefault = 128;  // walk group: cells of at most 128 particles
 constexpr double kWalkFarRatio = 0.4;  // far cells: box half diagonal < 0.4 theta |R|
@@ -158,7 +159,8 @@ struct nb::BHWorkspace {
     DevBuf lvl_leaf, lvl_sh, blkcnt, blkoff, levels, leaf_of, counters;
     DevBuf n_pstart, n_pend, n_child, n_cend, n_level, n_mom, w_mac, w_link, w_q0, w_q1;
     DevBuf walk_scratch, gflag, gstart, glev, totals, sortst, gorder, leaves;
-    int walk_blocks = 0;
+    int walk_blocks = 0;  // blocks the frontier scratch is sized for
+    int cus = 0;
     int group_size = 0;
     int sort_slots = 0, sort_slots_key = 0;  // radix-pass blocks resident at once, for which shape
     // Node counts per level of a recent evaluation, copied to pinned host
@@ -1313,14 +1315,15 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
     }
     __shared__ float4 pl[128];                 // staged source particles
     __shared__ float4 sc0[64], sc1[64], sc2[64];  // staged cells: centre of mass, (M Qxx Qyy Qzz), (Qxy Qxz Qyz)
-    __shared__ int2 fl[2][64];                 // frontier entries 0..63 of this level / the next
+    constexpr int kFl = 64;
+    __shared__ int2 fl[2][kFl];                // frontier entries 0..63 of this level / the next
     __shared__ int flist[kFar ? 64 : 1];       // staged far cells (node ids)
     __shared__ float lexp[kFar ? 24 : 1];      // the group's local expansion coefficients
     __shared__ int sgroup;
 
     const int lane = threadIdx.x;
     const unsigned long long lt = lanemask_lt();
-    // Frontier entries from 64 on live in a per-block slice of global scratch.
+    // Frontier entries from kFl on live in a per-block slice of global scratch.
     int2* fr0 = scratch + (size_t)blockIdx.x * 2 * fcap;
     int2* fr1 = fr0 + fcap;
     const int ngroups = li->n_groups;
@@ -1629,7 +1632,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             const int no = __popcll(bo);
             if (open) {
                 const int pos = nnext + __popcll(bo & lt);
-                if (pos < 64) fl[nb][pos] = lk;
+                if (pos < kFl) fl[nb][pos] = lk;
                 else if (pos < fcap) nxtg[pos] = lk;
             }
             if (nnext + no > fcap) gov = true;
@@ -1752,7 +1755,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                 int id = -1;
                 if (ps < ncur) {
                     int2 rg;
-                    if (base < 64) {  // base is a multiple of 8: ps < 64
+                    if (base < kFl) {  // base is a multiple of 8: ps < kFl
                         rg = fl[cb][ps];
                     } else {  // (an atomic load keeps the two reads from merging into a flat load)
                         const unsigned long long v = __hip_atomic_load(
@@ -2202,28 +2205,30 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     if (ev && ev[6]) NB_HIP(hipEventRecord(ev[6], s));
 
     const int ngroups = grid_for(n, 16);  // upper bound on walk groups in flight
-    if (w->walk_blocks == 0) {
-        int dev = 0, cus = 0;
+    const bool quad = multipole == NB_QUADRUPOLE;
+    const double rho = walk_far_ratio() * std::min(theta, 0.5);  // rho <= 0.2 by default
+    const float rho2 = (float)(rho * rho);
+    const bool far = quad && rho2 > 0.f;
+    if (w->cus == 0) {
+        int dev = 0;
         NB_HIP(hipGetDevice(&dev));
-        NB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        int bpc = kWalkBlocksPerCU;
-        if (const char* e = std::getenv("NB_WALK_BPC")) bpc = std::max(1, std::min(64, std::atoi(e)));
-        w->walk_blocks = std::max(1, cus) * bpc;
+        NB_HIP(hipDeviceGetAttribute(&w->cus, hipDeviceAttributeMultiprocessorCount, dev));
+        w->cus = std::max(1, w->cus);
     }
-    const int wblocks = std::min(ngroups, w->walk_blocks);
+    int bpc = far ? kWalkBlocksPerCUFar : kWalkBlocksPerCU;
+    if (const char* e = std::getenv("NB_WALK_BPC")) bpc = std::max(1, std::min(64, std::atoi(e)));
+    // The frontier scratch serves the largest grid either walk variant uses.
+    w->walk_blocks = std::max(w->walk_blocks, w->cus * std::max(bpc, kWalkBlocksPerCU));
+    const int wblocks = std::min(ngroups, w->cus * bpc);
     const int fcap = walk_fcap(theta);
     NB_TRY(w->walk_scratch.reserve((size_t)w->walk_blocks * 2 * fcap * sizeof(int2)));
     const float s2 = (float)soft2;
-    const bool quad = multipole == NB_QUADRUPOLE;
 #define NB_WALK(Q, E, F)                                                                       \
     hipLaunchKernelGGL((k_walk<Q, E, F>), dim3(wblocks), dim3(64), 0, s, n,                      \
                        w->spos.as<float4>(), v0, w->w_mac.as<float4>(), w->w_link.as<int2>(),   \
                        w->w_q0.as<float4>(), w->w_q1.as<float4>(), s2, G, ax, ay, az,          \
                        w->walk_scratch.as<int2>(), fcap, w->gstart.as<int>(),                \
                        w->gorder.as<int>(), li, ctr, rho2)
-    const double rho = walk_far_ratio() * std::min(theta, 0.5);  // rho <= 0.2 by default
-    const float rho2 = (float)(rho * rho);
-    const bool far = quad && rho2 > 0.f;
     if (quad) {
         if (s2 > 0.f) {
             if (far) NB_WALK(true, false, true);
