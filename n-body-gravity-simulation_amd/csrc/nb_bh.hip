@@ -1376,7 +1376,8 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                     bh2 = fmaxf(hi2 - bc2, bc2 - lo2);
         // Far-cell test: h^2 < rho2 |R|^2 with h the box's half diagonal.
         const float rg2 = fmaf(bh0, bh0, fmaf(bh1, bh1, bh2 * bh2));
-        const float hg = sqrtf(rg2), hinv = hg > 0.f ? 1.f / hg : 0.f;
+        // (A box below 1e-30 counts as a point: its orders 1-3 vanish against order 0.)
+        const float hg = sqrtf(rg2), hinv = hg > 1e-30f ? 1.f / hg : 0.f;
         int nfar = 0;
         unsigned long long npf = 0;
         if (kFar) {
