@@ -834,3 +834,22 @@ def test_monopole_walk_has_no_far_cells(nb, monkeypatch):
     monkeypatch.setenv("NB_WALK_FAR", "0")
     g, q = run_bh(nb, pl, eps=0.005, mp="monopole", G=1.0)
     assert f.stats()["far_cells"] == 0 and np.array_equal(a, np.stack(q.acc()))
+
+
+def test_far_expansion_of_one_particle_groups_is_the_per_target_sum(nb, monkeypatch):
+    """With leaves and walk groups of one particle the group box has zero
+    size: every accepted cell is far and the expansion about the particle
+    reduces to its order 0, the softened monopole + quadrupole field at the
+    particle -- the per-target term itself.  Far on and far off then differ
+    only by FP32 summation order."""
+    pl = nb.ic_plummer(16384, 9)
+    monkeypatch.setenv("NB_WALK_GROUP", "1")
+    monkeypatch.setenv("NB_WALK_FAR", "0")
+    f0, p0 = run_bh(nb, pl, eps=0.005, cap=1, G=1.0)
+    monkeypatch.delenv("NB_WALK_FAR")
+    f1, p1 = run_bh(nb, pl, eps=0.005, cap=1, G=1.0)
+    s0, s1 = f0.stats(), f1.stats()
+    assert s0["n_groups"] == 16384 and s1["far_cells"] == s0["pc_interactions"]
+    assert s1["pc_interactions"] == 0 and s1["pp_interactions"] == s0["pp_interactions"]
+    e = rel_err(p1.acc(), p0.acc())
+    assert e.max() < 2e-5 and np.median(e) < 1e-6, (np.median(e), e.max())
