@@ -48,7 +48,8 @@ int nb_device_count(int* count);
 int nb_set_device(int device);
 /* Waits for all outstanding work on the current device. */
 int nb_synchronize(void);
-/* Library version string. */
+/* Library version string.  0.2: nb_bh_stats gained far_cells and
+ * far_target_terms (callers built against 0.1's struct must be rebuilt). */
 const char* nb_version(void);
 
 /* ---------------------------------------------------------- particle state */

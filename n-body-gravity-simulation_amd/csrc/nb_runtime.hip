@@ -196,7 +196,7 @@ using namespace nb;
 
 extern "C" {
 
-const char* nb_version(void) { return "nbody_amd 0.1 (gfx950)"; }
+const char* nb_version(void) { return "nbody_amd 0.2 (gfx950)"; }
 
 int nb_device_count(int* count) {
     if (!count) {
