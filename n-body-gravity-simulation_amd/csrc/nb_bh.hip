@@ -514,7 +514,8 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
     int n, const unsigned long long* __restrict__ kin, const unsigned* __restrict__ vin,
     unsigned long long* __restrict__ kout, unsigned* __restrict__ vout, int pass,
     SortState* __restrict__ ss, unsigned* __restrict__ look, Counters* __restrict__ ctr,
-    const float4* __restrict__ upos, float4* __restrict__ spos, int resident) {
+    const float4* __restrict__ upos, float4* __restrict__ spos, int resident,
+    unsigned* __restrict__ gtot) {
     constexpr int kOsTile = kOsThreads * kSortRows;
     constexpr int kWaves = kOsThreads / 64;
     static_assert(kOsThreads >= kSortBins, "one owner thread per digit");
@@ -603,12 +604,59 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(
             loff += sred[0][k];
             dstart += sred[1][k];
         }
-        // Decoupled look-back for digit t, kLookWindow earlier tiles per
-        // memory round trip: their words are consumed nearest first until a
-        // tile with an inclusive prefix; an unpublished word stalls the walk,
-        // which resumes at that tile.
         int excl = 0;
-        if (tile > 0) {
+        if (gtot) {
+            // Every tile holds a slot: a two-level prefix.  The counts of this
+            // tile's earlier group mates (groups of kLookWindow tiles), then the
+            // totals of the earlier groups, each published by its group's last
+            // tile once it has its own in-group prefix.  No tile waits on a
+            // later one, so the waits end; each level takes a memory round
+            // trip per kLookWindow words.
+            const int grp = tile / kLookWindow, first = grp * kLookWindow;
+            // Sum of cnt published words p0[j * kSortBins], j < cnt.
+            auto sum_published = [&](const unsigned* p0, int cnt) {
+                int sum = 0;
+                for (int b0 = 0; b0 < cnt && !sbad; b0 += kLookWindow) {
+                    const int m = min(kLookWindow, cnt - b0);
+                    for (int spins = 0;; ++spins) {
+                        unsigned v[kLookWindow];
+#pragma unroll
+                        for (int j = 0; j < kLookWindow; ++j)
+                            v[j] = j < m ? __hip_atomic_load(p0 + (size_t)(b0 + j) * kSortBins,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                         : (1u << 30);
+                        bool ready = true;
+                        int part = 0;
+#pragma unroll
+                        for (int j = 0; j < kLookWindow; ++j) {
+                            ready = ready && (v[j] >> 30) != 0u;
+                            part += (int)(v[j] & 0x3fffffffu);
+                        }
+                        if (ready) {
+                            sum += part;
+                            break;
+                        }
+                        if (spins > kLookbackSpins) {
+                            raise_flag(ctr, 8);
+                            sbad = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                return sum;
+            };
+            const int in_grp = sum_published(look + (size_t)first * kSortBins + t, tile - first);
+            if (tile == first + kLookWindow - 1 || tile == (int)gridDim.x - 1)
+                __hip_atomic_store(gtot + (size_t)grp * kSortBins + t,
+                                   (1u << 30) | (unsigned)(in_grp + c), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            excl = in_grp + sum_published(gtot + t, grp);
+        } else if (tile > 0) {
+            // Decoupled look-back for digit t, kLookWindow earlier tiles per
+            // memory round trip: their words are consumed nearest first until a
+            // tile with an inclusive prefix; an unpublished word stalls the walk,
+            // which resumes at that tile.
             int tt = tile - 1;
             int spins = 0;
             bool done = false;
@@ -1955,7 +2003,7 @@ int64_t bh_workspace_bytes(int64_t n, int leaf_cap, double theta, int cus) {
     const int64_t per_particle = 8 + 8 + 4 + 4 + 16 + 16 + 1 + 1 + 1 + 4 + 1 + 4 + 4 + 4;
     const int64_t per_node = 4 * 4 + 1 + 16 + 8 + 16 + 16;
     return n * per_particle + 4 + ncap * per_node + std::min<int64_t>(ncap, kMomNodes) * 80 +
-           nsort * kSortBins * kSortPasses * 4 +
+           (nsort + (nsort + 7) / 8) * kSortBins * kSortPasses * 4 +
            (int64_t)sizeof(SortState) + nblk * kCols * 4 +
            (int64_t)std::max(kSortBins, kCols) * 4 + 1024 * 6 * 8 + 4096 +
            walk_scratch_bytes(theta, cus);
@@ -1999,7 +2047,9 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     NB_TRY(w->keys1.reserve((size_t)n * 8));
     NB_TRY(w->vals0.reserve((size_t)n * 4));
     NB_TRY(w->vals1.reserve((size_t)n * 4));
-    NB_TRY(w->hist.reserve((size_t)nsort * kSortBins * kSortPasses * 4));  // look-back words
+    const int ngrp = (nsort + win - 1) / win;  // tile groups of a resident pass
+    // look-back words, then the tile groups' totals
+    NB_TRY(w->hist.reserve((size_t)(nsort + ngrp) * kSortBins * kSortPasses * 4));
     NB_TRY(w->sortst.reserve(sizeof(SortState)));
     NB_TRY(w->totals.reserve((size_t)std::max(kSortBins, kCols) * 4));
     NB_TRY(w->spos.reserve((size_t)n * 16));
@@ -2040,7 +2090,7 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
                        w->bbox_part.as<double>(), ctr, ss);
     if (ev && ev[1]) NB_HIP(hipEventRecord(ev[1], s));
     unsigned* look = w->hist.as<unsigned>();
-    const int nlook = nsort * kSortBins * kSortPasses;
+    const int nlook = (nsort + ngrp) * kSortBins * kSortPasses;  // zeroed by the key kernel
     // Key grid: fewer blocks for small n (each block adds its digit table
     // to the global counts once); NB_KEY_BLOCKS (tuning) overrides.
     int kblocks = n <= (1 << 22) ? kKeyBlocks / 2 : kKeyBlocks;
@@ -2072,11 +2122,12 @@ int bh_eval(BHWorkspace* w, double G, double soft2, double theta, int leaf_cap, 
     }
     for (int p = 0; p < kSortPasses; ++p) {
         unsigned* lp = look + (size_t)p * nsort * kSortBins;
+        unsigned* gp = look + (size_t)kSortPasses * nsort * kSortBins + (size_t)p * ngrp * kSortBins;
         float4* gath = p == kSortPasses - 1 ? w->spos.as<float4>() : nullptr;
         const int resident = nsort <= w->sort_slots / 2 ? 1 : 0;  // half: margin for other work
 #define NB_PASS(T, R, W)                                                                         \
     hipLaunchKernelGGL((k_onesweep<T, R, W>), dim3(nsort), dim3(T), 0, s, n, k0, v0, k1, v1, p, \
-                       ss, lp, ctr, w->upos.as<float4>(), gath, resident)
+                       ss, lp, ctr, w->upos.as<float4>(), gath, resident, resident ? gp : nullptr)
         switch (tile * 64 + win) {
             case 4096 * 64 + 16: NB_PASS(512, 8, 16); break;
             case 4096 * 64 + 32: NB_PASS(512, 8, 32); break;
