@@ -40,8 +40,8 @@
 //             walk record is read once per candidate; accepted cells go from
 //             registers to an LDS staging list, opened leaves' particles to a
 //             second one, and each list is evaluated by all lanes (two targets
-//             per lane) in batches of 60-64 entries, a multiple of the number
-//             of lane subsets.  The first 64 frontier entries of a level are
+//             per lane, or three where that fills more lanes) in batches of
+//             60-64 entries, a multiple of the number of lane subsets.  The first 64 frontier entries of a level are
 //             kept in LDS.  An accepted cell far from the group compared with
 //             the group's size (quadrupole walk) instead enters a third-order
 //             local expansion about the group centre, evaluated once per
@@ -91,9 +91,10 @@ constexpr int kLookbackSpins = 1 << 22;  // polls before a tile gives up waiting
 constexpr int kWalkFcap = 2048;
 constexpr int kWalkFcapMax = 65536;
 constexpr int kWalkBlocksPerCU = 24;  // persistent one-wave walk blocks per CU (NB_WALK_BPC overrides)
-constexpr int kWalkBlocksPerCUFar = 20;  // the same for the far-cell walk (measured best)
-constexpr int kGroup = 128;     // largest walk group (a wave holds two targets per lane)
+constexpr int kWalkBlocksPerCUFar = 20;  // the same for the far-cell walk (5 waves per SIMD)
+constexpr int kGroup = 128;     // largest walk group (a wave holds two or three targets per lane)
 constexpr int kGroupDefault = 128;  // walk group: cells of at most 128 particles
+constexpr int kFarBatch = 128;  // far cells per expansion batch: two per lane, one reduction
 constexpr double kWalkFarRatio = 0.4;  // far cells: box half diagonal < 0.4 theta |R|
 constexpr int64_t kMaxNodes = (int64_t)1 << 28;  // node pool limit (32-bit record offsets in the walk)
 // Per-block count columns: 22 levels, group starts, leaf starts, oversize leaves.
@@ -1331,13 +1332,14 @@ __global__ __launch_bounds__(kBandThreads) void k_upsweep_band(
 // monopole + quadrupole field gradient and L3, L4 the monopole's higher
 // derivatives (-M d^n(1/r)/dR^n, traceless, unsoftened).  L2..L4 are stored
 // scaled by h^(n-1) against d/h, so every stored number is bounded by the
-// cell's M/r^2.  Far cells are staged by node id, 64 at a time; each lane
-// computes one cell's 24 coefficients (3 + 5 + 7 + 9 independent components),
+// cell's M/r^2.  Far cells are staged by node id, 128 at a time; each lane
+// adds up two cells' 24 coefficients (3 + 5 + 7 + 9 independent components),
 // a fixed-order reduce-scatter sums them over the wave and eight lanes add the
 // sums to the group's LDS accumulators.  Each target evaluates the expansion
-// once at the end.
+// once at the end.  The far-cell walk runs 5 waves per SIMD (96 VGPRs): the
+// extra registers pay for the third target and the second far cell per lane.
 template <bool kQuad, bool kEps0, bool kFar>
-__global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict__ spos,
+__global__ __launch_bounds__(64, kFar ? 5 : 6) void k_walk(int n, const float4* __restrict__ spos,
                                              const unsigned* __restrict__ perm,
                                              const float4* __restrict__ mac,
                                              const int2* __restrict__ link,
@@ -1365,7 +1367,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
     __shared__ float4 sc0[64], sc1[64], sc2[64];  // staged cells: centre of mass, (M Qxx Qyy Qzz), (Qxy Qxz Qyz)
     constexpr int kFl = 64;
     __shared__ int2 fl[2][kFl];                // frontier entries 0..63 of this level / the next
-    __shared__ int flist[kFar ? 64 : 1];       // staged far cells (node ids)
+    __shared__ int flist[kFar ? kFarBatch : 1];  // staged far cells (node ids)
     __shared__ float lexp[kFar ? 24 : 1];      // the group's local expansion coefficients
     __shared__ int sgroup;
 
@@ -1392,24 +1394,33 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
       for (int g0 = gstart[grp]; g0 < gend; g0 += 128) {  // oversize leaves in chunks of 128
         const int nact = min(128, gend - g0);
         // Every lane holds two targets, A = tix and B = tix + nl, of a subset of
-        // nl = ceil(nact/2) lanes.  The wave runs nsub = 64/nl such subsets; each
-        // takes every nsub-th entry of the interaction lists and the partial
-        // sums are added in subset order at the end.
-        const int nl = (nact + 1) >> 1;
+        // nl = ceil(nact/2) lanes, or (far-cell walk, t3) three, + C = tix + 2 nl,
+        // of ceil(nact/3) lanes where that keeps more (target, entry) slots busy:
+        // nsub3 nact of 192 against nsub2 nact of 128 per trip.  The wave runs
+        // nsub = 64/nl such subsets; each takes every nsub-th entry of the
+        // interaction lists and the partial sums are added in subset order at
+        // the end.
+        const int nl2 = (nact + 1) >> 1, nl3 = (nact + 2) / 3;
+        const bool t3 = kFar && 2 * (64 / nl3) > 3 * (64 / nl2);
+        const int nl = t3 ? nl3 : nl2;
         const int nsub = 64 / nl;
         const int cfull = nsub * (64 / nsub);  // staged cells (particles) per evaluation batch, >= 60
         const int sub = lane / nl;
         const int tix = lane - sub * nl;
         const bool live = sub < nsub;
         const bool hasB = live && tix + nl < nact;
+        const bool hasC = t3 && live && tix + 2 * nl < nact;
         const bool lead = live && sub == 0;
-        const int kA = g0 + tix, kB = g0 + tix + nl;
+        const int kA = g0 + tix, kB = g0 + tix + nl, kC = g0 + tix + 2 * nl;
         const float4 meA = spos[live ? kA : g0];
         const float4 meB = hasB ? spos[kB] : meA;  // a lane without B computes A twice
+        const float4 meC = hasC ? spos[kC] : meA;  // likewise C
 
         // Group bounding box.
-        float lo0 = fminf(meA.x, meB.x), lo1 = fminf(meA.y, meB.y), lo2 = fminf(meA.z, meB.z);
-        float hi0 = fmaxf(meA.x, meB.x), hi1 = fmaxf(meA.y, meB.y), hi2 = fmaxf(meA.z, meB.z);
+        float lo0 = fminf(fminf(meA.x, meB.x), meC.x), lo1 = fminf(fminf(meA.y, meB.y), meC.y),
+              lo2 = fminf(fminf(meA.z, meB.z), meC.z);
+        float hi0 = fmaxf(fmaxf(meA.x, meB.x), meC.x), hi1 = fmaxf(fmaxf(meA.y, meB.y), meC.y),
+              hi2 = fmaxf(fmaxf(meA.z, meB.z), meC.z);
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             lo0 = fminf(lo0, __shfl_xor(lo0, d, 64));
@@ -1434,6 +1445,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
         }
 
         float fxA = 0.f, fyA = 0.f, fzA = 0.f, fxB = 0.f, fyB = 0.f, fzB = 0.f;
+        float fxC = 0.f, fyC = 0.f, fzC = 0.f;
         int ncl = 0, npl = 0, nnext = 0, nb = 0;
         bool gov = false;  // this group's frontier overflowed: its sums are incomplete
         unsigned long long npp = 0, npc = 0;
@@ -1470,10 +1482,19 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                 __syncthreads();
             }
             int j = jfirst;
-            for (int t = 0; t < trips; ++t, j += nsub) {
-                const float4 s = pl[j];
-                pp_one(s, meA, fxA, fyA, fzA);
-                pp_one(s, meB, fxB, fyB, fzB);
+            if (t3) {
+                for (int t = 0; t < trips; ++t, j += nsub) {
+                    const float4 s = pl[j];
+                    pp_one(s, meA, fxA, fyA, fzA);
+                    pp_one(s, meB, fxB, fyB, fzB);
+                    pp_one(s, meC, fxC, fyC, fzC);
+                }
+            } else {
+                for (int t = 0; t < trips; ++t, j += nsub) {
+                    const float4 s = pl[j];
+                    pp_one(s, meA, fxA, fyA, fzA);
+                    pp_one(s, meB, fxB, fyB, fzB);
+                }
             }
             npp += (unsigned long long)cnt;
         };
@@ -1521,104 +1542,127 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             }
             __syncthreads();
             int j = jfirst;
-            for (int t = 0; t < trips; ++t, j += nsub) {
-                const float4 c = sc0[j];
-                const float4 qa = sc1[j];
-                const float4 qb = kQuad ? sc2[j] : qa;
-                pc_one(c, qa, qb, meA, fxA, fyA, fzA);
-                pc_one(c, qa, qb, meB, fxB, fyB, fzB);
+            if (t3) {
+                for (int t = 0; t < trips; ++t, j += nsub) {
+                    const float4 c = sc0[j];
+                    const float4 qa = sc1[j];
+                    const float4 qb = kQuad ? sc2[j] : qa;
+                    pc_one(c, qa, qb, meA, fxA, fyA, fzA);
+                    pc_one(c, qa, qb, meB, fxB, fyB, fzB);
+                    pc_one(c, qa, qb, meC, fxC, fyC, fzC);
+                }
+            } else {
+                for (int t = 0; t < trips; ++t, j += nsub) {
+                    const float4 c = sc0[j];
+                    const float4 qa = sc1[j];
+                    const float4 qb = kQuad ? sc2[j] : qa;
+                    pc_one(c, qa, qb, meA, fxA, fyA, fzA);
+                    pc_one(c, qa, qb, meB, fxB, fyB, fzB);
+                }
             }
             npc += (unsigned long long)cnt;
             __syncthreads();
         };
 
-        // Adds the local expansion terms of the cnt (<= 64) staged far cells,
-        // one per lane, to lexp (a lane past cnt evaluates a massless cell at
-        // unit distance, which adds exact zeros); the staging slots are free
-        // afterwards.
+        // Adds the local expansion terms of the cnt (<= kFarBatch) staged far
+        // cells to lexp: lane l sums cells l and l + 64 (a lane past cnt
+        // evaluates a massless cell at unit distance, which adds exact zeros),
+        // then the wave sums the lanes.  The staging slots are free afterwards.
         auto flush_far = [&](int cnt) {
             __syncthreads();
-            const bool valid = lane < cnt;
-            const int fid = valid ? flist[lane] : 0;
-            __syncthreads();
-            const unsigned o16 = (unsigned)fid << 4;
-            const float4 mc = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(mac) + o16);
-            float4 qa = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(q0) + o16);
-            float4 qb = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(q1) + o16);
-            float x = mc.x - bc0, y = mc.y - bc1, z = mc.z - bc2;
-            if (!valid) {
-                x = 1.f;
-                y = 0.f;
-                z = 0.f;
-                qa = make_float4(0.f, 0.f, 0.f, 0.f);
-                qb = qa;
+            const int trips = __builtin_amdgcn_readfirstlane((cnt + 63) >> 6);
+            int fids[kFarBatch / 64];
+#pragma unroll
+            for (int u = 0; u < kFarBatch / 64; ++u) {
+                const int slot = u * 64 + lane;
+                fids[u] = slot < cnt ? flist[slot] : -1;
             }
-            if (!kQuad) qb = make_float4(0.f, 0.f, 0.f, 0.f);
+            __syncthreads();
             float c[24];
-            const float M = qa.x;
-            const float Qxx = kQuad ? qa.y : 0.f, Qyy = kQuad ? qa.z : 0.f, Qzz = kQuad ? qa.w : 0.f;
-            // Order 0: softened monopole + quadrupole acceleration at the box centre.
-            const float r2s = fmaf(x, x, fmaf(y, y, fmaf(z, z, soft2)));
-            const float is = __builtin_amdgcn_rsqf(r2s);
-            const float is2 = is * is;
-            const float qdx = fmaf(Qxx, x, fmaf(qb.x, y, qb.y * z));
-            const float qdy = fmaf(qb.x, x, fmaf(Qyy, y, qb.z * z));
-            const float qdz = fmaf(qb.y, x, fmaf(qb.z, y, Qzz * z));
-            const float dqd = fmaf(x, qdx, fmaf(y, qdy, z * qdz));
-            const float is5 = (is2 * is2) * is;
-            const float sc = fmaf(2.5f, dqd * is2, M * r2s);
-            c[0] = fmaf(x, sc, -qdx) * is5;
-            c[1] = fmaf(y, sc, -qdy) * is5;
-            c[2] = fmaf(z, sc, -qdz) * is5;
-            // Orders 1-3, unsoftened, in the unit vector n = R/r: order n is
-            // stored times h^(n-1).  g2 = 1/r^2, hw = h/r.
-            const float r2 = fmaf(x, x, fmaf(y, y, z * z));
-            const float w = __builtin_amdgcn_rsqf(r2);
-            const float g2 = w * w;
-            const float nx = x * w, ny = y * w, nz = z * w;
-            const float hw = hg * w;
-            const float k2 = hw * g2;      // h/r^3
-            const float k3 = M * hw * k2;  // M h^2/r^4
-            const float k4 = k3 * hw;      // M h^3/r^5
-            // L2 h = (h/r^3) [(M + 5/2 s) I - (3M + 35/2 s) nn + 5 (n p + p n) - Q/r^2],
-            // p = Qn/r^2, s = n.p (traceless: xx, yy, xy, xz, yz).
-            const float sig = dqd * (g2 * g2);
-            const float wg = w * g2;
-            const float px = qdx * wg, py = qdy * wg, pz = qdz * wg;
-            const float A = -fmaf(17.5f, sig, 3.f * M) * k2;
-            const float C = fmaf(2.5f, sig, M) * k2;
-            const float B = 5.f * k2;
-            const float D = -k2 * g2;
-            c[3] = fmaf(A, nx * nx, fmaf(2.f * B, nx * px, fmaf(D, Qxx, C)));
-            c[4] = fmaf(A, ny * ny, fmaf(2.f * B, ny * py, fmaf(D, Qyy, C)));
-            c[5] = fmaf(A, nx * ny, fmaf(B, fmaf(nx, py, ny * px), D * qb.x));
-            c[6] = fmaf(A, nx * nz, fmaf(B, fmaf(nx, pz, nz * px), D * qb.y));
-            c[7] = fmaf(A, ny * nz, fmaf(B, fmaf(ny, pz, nz * py), D * qb.z));
-            // L3 h^2 = (M h^2/r^4) (15 nnn - 3 sym(n I)): xxx, xxy, xxz, xyy, xyz, yyy, yyz.
-            const float e = 15.f * k3, f = 3.f * k3;
-            const float xx = nx * nx, yy = ny * ny;
-            const float ex = fmaf(e, xx, -f), ey = fmaf(e, yy, -f);
-            c[8] = nx * fmaf(-2.f, f, ex);
-            c[9] = ny * ex;
-            c[10] = nz * ex;
-            c[11] = nx * ey;
-            c[12] = e * (nx * ny) * nz;
-            c[13] = ny * fmaf(-2.f, f, ey);
-            c[14] = nz * ey;
-            // L4 h^3 = (M h^3/r^5) (-105 nnnn + 15 sym(nn I) - 3 sym(I I)):
-            // xxxx, xxxy, xxxz, xxyy, xxyz, xyyy, xyyz, yyyy, yyyz.
-            const float a105 = -105.f * k4;
-            const float ux = fmaf(a105, xx, 45.f * k4), uy = fmaf(a105, yy, 45.f * k4);
-            const float vx = fmaf(a105, xx, 15.f * k4), vy = fmaf(a105, yy, 15.f * k4);
-            c[15] = fmaf(xx, fmaf(a105, xx, 90.f * k4), -9.f * k4);
-            c[16] = (nx * ny) * ux;
-            c[17] = (nx * nz) * ux;
-            c[18] = fmaf(xx, fmaf(a105, yy, 15.f * k4), fmaf(15.f * k4, yy, -3.f * k4));
-            c[19] = (ny * nz) * vx;
-            c[20] = (nx * ny) * uy;
-            c[21] = (nx * nz) * vy;
-            c[22] = fmaf(yy, fmaf(a105, yy, 90.f * k4), -9.f * k4);
-            c[23] = (ny * nz) * uy;
+#pragma unroll
+            for (int j = 0; j < 24; ++j) c[j] = 0.f;
+#pragma unroll
+            for (int u = 0; u < kFarBatch / 64; ++u) {
+                if (u >= trips) break;
+                const bool valid = fids[u] >= 0;
+                const unsigned o16 = (unsigned)(valid ? fids[u] : 0) << 4;
+                const float4 mc = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(mac) + o16);
+                float4 qa = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(q0) + o16);
+                float4 qb = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(q1) + o16);
+                float x = mc.x - bc0, y = mc.y - bc1, z = mc.z - bc2;
+                if (!valid) {
+                    x = 1.f;
+                    y = 0.f;
+                    z = 0.f;
+                    qa = make_float4(0.f, 0.f, 0.f, 0.f);
+                    qb = qa;
+                }
+                if (!kQuad) qb = make_float4(0.f, 0.f, 0.f, 0.f);
+                const float M = qa.x;
+                const float Qxx = kQuad ? qa.y : 0.f, Qyy = kQuad ? qa.z : 0.f, Qzz = kQuad ? qa.w : 0.f;
+                // Order 0: softened monopole + quadrupole acceleration at the box centre.
+                const float r2s = fmaf(x, x, fmaf(y, y, fmaf(z, z, soft2)));
+                const float is = __builtin_amdgcn_rsqf(r2s);
+                const float is2 = is * is;
+                const float qdx = fmaf(Qxx, x, fmaf(qb.x, y, qb.y * z));
+                const float qdy = fmaf(qb.x, x, fmaf(Qyy, y, qb.z * z));
+                const float qdz = fmaf(qb.y, x, fmaf(qb.z, y, Qzz * z));
+                const float dqd = fmaf(x, qdx, fmaf(y, qdy, z * qdz));
+                const float is5 = (is2 * is2) * is;
+                const float sc = fmaf(2.5f, dqd * is2, M * r2s);
+                c[0] += fmaf(x, sc, -qdx) * is5;
+                c[1] += fmaf(y, sc, -qdy) * is5;
+                c[2] += fmaf(z, sc, -qdz) * is5;
+                // Orders 1-3, unsoftened, in the unit vector n = R/r: order n is
+                // stored times h^(n-1).  g2 = 1/r^2, hw = h/r.
+                const float r2 = fmaf(x, x, fmaf(y, y, z * z));
+                const float w = __builtin_amdgcn_rsqf(r2);
+                const float g2 = w * w;
+                const float nx = x * w, ny = y * w, nz = z * w;
+                const float hw = hg * w;
+                const float k2 = hw * g2;      // h/r^3
+                const float k3 = M * hw * k2;  // M h^2/r^4
+                const float k4 = k3 * hw;      // M h^3/r^5
+                // L2 h = (h/r^3) [(M + 5/2 s) I - (3M + 35/2 s) nn + 5 (n p + p n) - Q/r^2],
+                // p = Qn/r^2, s = n.p (traceless: xx, yy, xy, xz, yz).
+                const float sig = dqd * (g2 * g2);
+                const float wg = w * g2;
+                const float px = qdx * wg, py = qdy * wg, pz = qdz * wg;
+                const float A = -fmaf(17.5f, sig, 3.f * M) * k2;
+                const float C = fmaf(2.5f, sig, M) * k2;
+                const float B = 5.f * k2;
+                const float D = -k2 * g2;
+                c[3] += fmaf(A, nx * nx, fmaf(2.f * B, nx * px, fmaf(D, Qxx, C)));
+                c[4] += fmaf(A, ny * ny, fmaf(2.f * B, ny * py, fmaf(D, Qyy, C)));
+                c[5] += fmaf(A, nx * ny, fmaf(B, fmaf(nx, py, ny * px), D * qb.x));
+                c[6] += fmaf(A, nx * nz, fmaf(B, fmaf(nx, pz, nz * px), D * qb.y));
+                c[7] += fmaf(A, ny * nz, fmaf(B, fmaf(ny, pz, nz * py), D * qb.z));
+                // L3 h^2 = (M h^2/r^4) (15 nnn - 3 sym(n I)): xxx, xxy, xxz, xyy, xyz, yyy, yyz.
+                const float e = 15.f * k3, f = 3.f * k3;
+                const float xx = nx * nx, yy = ny * ny;
+                const float ex = fmaf(e, xx, -f), ey = fmaf(e, yy, -f);
+                c[8] += nx * fmaf(-2.f, f, ex);
+                c[9] += ny * ex;
+                c[10] += nz * ex;
+                c[11] += nx * ey;
+                c[12] += e * (nx * ny) * nz;
+                c[13] += ny * fmaf(-2.f, f, ey);
+                c[14] += nz * ey;
+                // L4 h^3 = (M h^3/r^5) (-105 nnnn + 15 sym(nn I) - 3 sym(I I)):
+                // xxxx, xxxy, xxxz, xxyy, xxyz, xyyy, xyyz, yyyy, yyyz.
+                const float a105 = -105.f * k4;
+                const float ux = fmaf(a105, xx, 45.f * k4), uy = fmaf(a105, yy, 45.f * k4);
+                const float vx = fmaf(a105, xx, 15.f * k4), vy = fmaf(a105, yy, 15.f * k4);
+                c[15] += fmaf(xx, fmaf(a105, xx, 90.f * k4), -9.f * k4);
+                c[16] += (nx * ny) * ux;
+                c[17] += (nx * nz) * ux;
+                c[18] += fmaf(xx, fmaf(a105, yy, 15.f * k4), fmaf(15.f * k4, yy, -3.f * k4));
+                c[19] += (ny * nz) * vx;
+                c[20] += (nx * ny) * uy;
+                c[21] += (nx * nz) * vy;
+                c[22] += fmaf(yy, fmaf(a105, yy, 90.f * k4), -9.f * k4);
+                c[23] += (ny * nz) * uy;
+            }
             // Reduce-scatter over the wave: three pair exchanges inside each
             // 16-lane row (row_mirror, row_half_mirror, quad_perm 3210) halve
             // the coefficients a lane holds, 24 -> 12 -> 6 -> 3; the pair i, i^1
@@ -1726,18 +1770,18 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                 }
             }
 
-            // Far cells: node ids staged 64 at a time; ids past a full batch
-            // wait in registers during its evaluation.
+            // Far cells: node ids staged kFarBatch at a time; ids past a full
+            // batch wait in registers during its evaluation.
             if (kFar) {
                 const unsigned long long bf = __ballot(far);
                 if (bf) {
-                    const int fpos = nfar + __popcll(bf & lt);  // < 64 + 63
-                    if (far && fpos < 64) flist[fpos] = idc;
+                    const int fpos = nfar + __popcll(bf & lt);  // < kFarBatch + 63
+                    if (far && fpos < kFarBatch) flist[fpos] = idc;
                     nfar += __popcll(bf);
-                    if (nfar >= 64) {
-                        flush_far(64);
-                        if (far && fpos >= 64) flist[fpos - 64] = idc;
-                        nfar -= 64;
+                    if (nfar >= kFarBatch) {
+                        flush_far(kFarBatch);
+                        if (far && fpos >= kFarBatch) flist[fpos - kFarBatch] = idc;
+                        nfar -= kFarBatch;
                     }
                 }
             }
@@ -1826,6 +1870,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
         if (nsub > 1) {  // reduce the subsets' partial sums in subset order
             sc0[lane] = make_float4(fxA, fyA, fzA, 0.f);
             sc1[lane] = make_float4(fxB, fyB, fzB, 0.f);
+            if (t3) sc2[lane] = make_float4(fxC, fyC, fzC, 0.f);
             __syncthreads();
             if (lead) {
                 for (int q = 1; q < nsub; ++q) {
@@ -1837,6 +1882,12 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                     fxB += b.x;
                     fyB += b.y;
                     fzB += b.z;
+                    if (t3) {
+                        const float4 cc = sc2[q * nl + tix];
+                        fxC += cc.x;
+                        fyC += cc.y;
+                        fzC += cc.z;
+                    }
                 }
             }
             __syncthreads();
@@ -1887,6 +1938,7 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
             if (lead) {
                 l2p(meA, fxA, fyA, fzA);
                 if (hasB) l2p(meB, fxB, fyB, fzB);
+                if (hasC) l2p(meC, fxC, fyC, fzC);
             }
         }
         // A group whose frontier overflowed skipped cells: its accelerations
@@ -1903,6 +1955,12 @@ __global__ __launch_bounds__(64, 6) void k_walk(int n, const float4* __restrict_
                 ax[jb] = gs * (double)fxB;
                 ay[jb] = gs * (double)fyB;
                 az[jb] = gs * (double)fzB;
+            }
+            if (hasC) {
+                const unsigned jc = perm[kC];
+                ax[jc] = gs * (double)fxC;
+                ay[jc] = gs * (double)fyC;
+                az[jc] = gs * (double)fzC;
             }
         }
         tot_pp += npp * (unsigned long long)nact;
