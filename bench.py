@@ -411,9 +411,11 @@ def bench_config1(args):
            "workload": (f"direct all-pairs FP64 (reference CPU arithmetic), N={n}, m=1/N, "
                         f"x~U[-1,1]^3, v~N(0,0.1^2) (seed 1), G=1, eps={eps}, leapfrog dt={dt} "
                         f"x {K} steps (cached acceleration: {K + 1} evaluations)"),
-           "roofline": {"kernel": "k_direct_f64_exact", "bound": "latency",
-                        "note": "N=1000: 4 workgroups per launch, far below one wave per CU; "
-                                "launch latency and the dependent step chain bound the run"},
+           "roofline": {"kernel": "k_direct_f64_exact_wave", "bound": "latency",
+                        "note": "N=1000: one wave per target (1000 waves), the terms in "
+                                "parallel and their in-order FP64 sums on three lanes: the "
+                                "dependent chain of 999 adds per component and the launch "
+                                "latency bound each evaluation"},
            "accuracy": {"energy_drift_rel": float(abs(E1 - E0) / abs(E0))}}
     if not args.no_cpu:
         O = checker()

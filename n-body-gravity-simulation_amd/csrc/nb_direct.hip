@@ -8,7 +8,8 @@
 //    (CPU/src/force_calculator_naive.cpp:20-30): per target, j ascending,
 //    j != i skipped, ((dx*dx+dy*dy)+dz*dz)+eps2, IEEE sqrt and division,
 //    (inv*inv)*inv, (G*m_j)*inv3, no FMA (the library is built with
-//    -ffp-contract=off).
+//    -ffp-contract=off).  Up to 24576 particles k_direct_f64_exact_wave
+//    gives the same bits with one wave per target.
 //  * k_direct_f64_fast: FP64 with v_rsq_f64 + one Newton step and FMAs.
 //  * k_direct_f32: FP32 with v_rsq_f32, 3 sub + 3 fma + rsq + 3 mul + 3 fma
 //    per interaction, R targets per lane to amortise the LDS broadcast.
@@ -22,6 +23,7 @@
 namespace nb {
 
 constexpr int kDirectBlock = 256;
+constexpr int kExactWaveMaxDefault = 24576;  // n up to which k_direct_f64_exact_wave runs
 
 __global__ __launch_bounds__(kDirectBlock) void k_direct_f64_exact(
     int n, const double* __restrict__ x, const double* __restrict__ y,
@@ -65,6 +67,69 @@ __global__ __launch_bounds__(kDirectBlock) void k_direct_f64_exact(
         ay[i] = fy;
         az[i] = fz;
     }
+}
+
+// The same sums for small n, where one target per lane leaves most of the GPU
+// idle and each lane's chain of IEEE sqrt and division sets the time: one
+// wave per target.  Lane k computes the terms of sources j0 + k (the same
+// arithmetic as above: ((dx*dx+dy*dy)+dz*dz)+eps2, 1/sqrt, (inv*inv)*inv,
+// (G*m_j)*inv3, dx*s); lanes 0, 1, 2 then add the x, y, z terms in ascending
+// j, one add instruction per term for all three.  The skipped j == i
+// (and j >= n) contribute +0.0, which leaves the running sum unchanged: it
+// starts at +0.0 and so is never -0.0.
+constexpr int kExactWaves = 4;  // targets per workgroup
+
+__global__ __launch_bounds__(kExactWaves * 64) void k_direct_f64_exact_wave(
+    int n, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ m, double G, double soft2,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
+    __shared__ double tt[kExactWaves][3][64];  // the chunk's terms, per component
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = min(lane, 2);  // lanes 0, 1, 2 sum the x, y, z components
+    const int i = blockIdx.x * kExactWaves + w;
+    if (i >= n) return;  // the whole wave: no workgroup barrier follows
+    const double xi = x[i], yi = y[i], zi = z[i];
+    // Term of source j (branch-free: an excluded j reads a valid index and
+    // its term is replaced by +0.0).
+    auto term = [&](int j, double& px, double& py, double& pz) {
+        const bool use = j < n && j != i;
+        const int jc = min(j, n - 1);
+        const double dx = x[jc] - xi;
+        const double dy = y[jc] - yi;
+        const double dz = z[jc] - zi;
+        const double d2 = dx * dx + dy * dy + dz * dz + soft2;
+        const double inv = 1.0 / sqrt(d2);
+        const double inv3 = inv * inv * inv;
+        const double s = (G * m[jc]) * inv3;
+        px = use ? dx * s : 0.0;
+        py = use ? dy * s : 0.0;
+        pz = use ? dz * s : 0.0;
+    };
+    double f = 0.0;
+    double px, py, pz;
+    term(lane, px, py, pz);
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        tt[w][0][lane] = px;
+        tt[w][1][lane] = py;
+        tt[w][2][lane] = pz;
+        __builtin_amdgcn_wave_barrier();
+        // The next chunk's terms do not depend on the sums below, so their
+        // long division and square-root chains can overlap the adds.
+        double qx, qy, qz;
+        term(j0 + 64 + lane, qx, qy, qz);
+        // One add per term advances all three sums (lanes 0, 1, 2).
+        if (j0 + 64 <= n) {
+#pragma unroll
+            for (int k = 0; k < 64; ++k) f = f + tt[w][c][k];
+        } else {
+            for (int k = 0; k < n - j0; ++k) f = f + tt[w][c][k];
+        }
+        __builtin_amdgcn_wave_barrier();
+        px = qx;
+        py = qy;
+        pz = qz;
+    }
+    if (lane < 3) (lane == 0 ? ax : lane == 1 ? ay : az)[i] = f;
 }
 
 template <bool kEps0>
@@ -214,6 +279,13 @@ static int direct_rows() {
     return rows;
 }
 
+// Largest n for the wave-per-target exact kernel (both kernels give the same
+// bits); NB_DIRECT_EXACT_WAVE_MAX overrides it (0: never).
+static int exact_wave_max() {
+    if (const char* e = std::getenv("NB_DIRECT_EXACT_WAVE_MAX")) return std::atoi(e);
+    return kExactWaveMaxDefault;
+}
+
 int direct_eval(int precision, double G, double soft2, int64_t n64, const double* m,
                 const double* x, const double* y, const double* z, double* ax, double* ay,
                 double* az, DirectWS& ws, hipStream_t s) {
@@ -226,8 +298,13 @@ int direct_eval(int precision, double G, double soft2, int64_t n64, const double
     const int grid = grid_for(n, kDirectBlock);
     switch (precision) {
         case NB_DIRECT_F64_EXACT:
-            hipLaunchKernelGGL(k_direct_f64_exact, dim3(grid), dim3(kDirectBlock), 0, s, n, x, y,
-                               z, m, G, soft2, ax, ay, az);
+            if (n <= exact_wave_max())
+                hipLaunchKernelGGL(k_direct_f64_exact_wave, dim3(grid_for(n, kExactWaves)),
+                                   dim3(kExactWaves * 64), 0, s, n, x, y, z, m, G, soft2, ax, ay,
+                                   az);
+            else
+                hipLaunchKernelGGL(k_direct_f64_exact, dim3(grid), dim3(kDirectBlock), 0, s, n, x,
+                                   y, z, m, G, soft2, ax, ay, az);
             break;
         case NB_DIRECT_F64:
             if (soft2 > 0.0)

@@ -43,6 +43,26 @@ def test_f64_exact_bitwise_vs_oracle_ragged(nb, O, n, eps):
         assert np.array_equal(u, v)
 
 
+@pytest.mark.parametrize("n", [2, 65, 1000, 4097])
+@pytest.mark.parametrize("eps", [0.0, 0.05])
+def test_f64_exact_wave_and_lane_kernels_agree_bitwise(nb, O, monkeypatch, n, eps):
+    """Up to 24576 particles the exact direct sum runs one wave per target
+    (the terms computed in parallel, added in ascending j by one lane); above
+    that, one lane per target.  NB_DIRECT_EXACT_WAVE_MAX=0 forces the latter:
+    both give the oracle's bits."""
+    rng = np.random.default_rng(n + 7)
+    m = rng.uniform(0.5, 2.0, n)
+    x, y, z = rng.normal(size=(3, n))
+    d = dict(m=m, x=x, y=y, z=z)
+    monkeypatch.setenv("NB_DIRECT_EXACT_WAVE_MAX", "0")
+    lane = gpu_force(nb, d, "f64_exact", eps, G=1.0)
+    monkeypatch.delenv("NB_DIRECT_EXACT_WAVE_MAX")
+    wave = gpu_force(nb, d, "f64_exact", eps, G=1.0)
+    r = O.direct(m, x, y, z, 1.0, eps * eps)
+    for u, v, w in zip(lane, wave, r):
+        assert np.array_equal(u, w) and np.array_equal(v, w)
+
+
 def test_single_particle_has_zero_acceleration(nb):
     for prec in ("f64_exact", "f64", "f32"):
         a = gpu_force(nb, dict(m=np.ones(1), x=np.ones(1), y=np.ones(1), z=np.ones(1)), prec, 0.0)
