@@ -46,8 +46,10 @@ __global__ __launch_bounds__(kDirectBlock) void k_direct_f64_exact(
         __syncthreads();
         const int cnt = min(kDirectBlock, n - t0);
         if (valid) {
+            // Branch-free (its own j adds +0.0, an identity for a sum that
+            // starts at +0.0), so the chains of consecutive sources overlap.
+#pragma unroll 4
             for (int k = 0; k < cnt; ++k) {
-                if (t0 + k == i) continue;
                 const double dx = sx[k] - xi;
                 const double dy = sy[k] - yi;
                 const double dz = sz[k] - zi;
@@ -55,9 +57,10 @@ __global__ __launch_bounds__(kDirectBlock) void k_direct_f64_exact(
                 const double inv = 1.0 / sqrt(d2);
                 const double inv3 = inv * inv * inv;
                 const double s = sgm[k] * inv3;
-                fx = fx + dx * s;
-                fy = fy + dy * s;
-                fz = fz + dz * s;
+                const bool self = t0 + k == i;
+                fx = fx + (self ? 0.0 : dx * s);
+                fy = fy + (self ? 0.0 : dy * s);
+                fz = fz + (self ? 0.0 : dz * s);
             }
         }
         __syncthreads();
