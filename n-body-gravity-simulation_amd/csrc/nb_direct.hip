@@ -8,7 +8,7 @@
 //    (CPU/src/force_calculator_naive.cpp:20-30): per target, j ascending,
 //    j != i skipped, ((dx*dx+dy*dy)+dz*dz)+eps2, IEEE sqrt and division,
 //    (inv*inv)*inv, (G*m_j)*inv3, no FMA (the library is built with
-//    -ffp-contract=off).  Up to 24576 particles k_direct_f64_exact_wave
+//    -ffp-contract=off).  Up to 20480 particles k_direct_f64_exact_wave
 //    gives the same bits with one wave per target.
 //  * k_direct_f64_fast: FP64 with v_rsq_f64 + one Newton step and FMAs.
 //  * k_direct_f32: FP32 with v_rsq_f32, 3 sub + 3 fma + rsq + 3 mul + 3 fma
@@ -23,7 +23,7 @@
 namespace nb {
 
 constexpr int kDirectBlock = 256;
-constexpr int kExactWaveMaxDefault = 24576;  // n up to which k_direct_f64_exact_wave runs
+constexpr int kExactWaveMaxDefault = 20480;  // n up to which k_direct_f64_exact_wave runs
 
 __global__ __launch_bounds__(kDirectBlock) void k_direct_f64_exact(
     int n, const double* __restrict__ x, const double* __restrict__ y,

@@ -46,7 +46,7 @@ def test_f64_exact_bitwise_vs_oracle_ragged(nb, O, n, eps):
 @pytest.mark.parametrize("n", [2, 65, 1000, 4097])
 @pytest.mark.parametrize("eps", [0.0, 0.05])
 def test_f64_exact_wave_and_lane_kernels_agree_bitwise(nb, O, monkeypatch, n, eps):
-    """Up to 24576 particles the exact direct sum runs one wave per target
+    """Up to 20480 particles the exact direct sum runs one wave per target
     (the terms computed in parallel, added in ascending j by one lane); above
     that, one lane per target.  NB_DIRECT_EXACT_WAVE_MAX=0 forces the latter:
     both give the oracle's bits."""
