@@ -853,3 +853,22 @@ def test_far_expansion_of_one_particle_groups_is_the_per_target_sum(nb, monkeypa
     assert s1["pc_interactions"] == 0 and s1["pp_interactions"] == s0["pp_interactions"]
     e = rel_err(p1.acc(), p0.acc())
     assert e.max() < 2e-5 and np.median(e) < 1e-6, (np.median(e), e.max())
+
+
+@pytest.mark.parametrize("mp", ["quadrupole", "monopole"])
+def test_walk_results_do_not_depend_on_the_schedule(nb, monkeypatch, mp):
+    """The persistent walk hands groups to waves from an atomic queue, so
+    which wave computes a group, and when, differs from run to run and with
+    the grid size.  Each group's sums run in a fixed order (staged batches,
+    subset order, the far cells' fixed-order reduce-scatter), so the
+    accelerations are bitwise the same on a repeat and with 1, 4 or the
+    default number of waves per CU."""
+    pl = nb.ic_plummer(65536, 13)
+    f, p = run_bh(nb, pl, eps=0.005, mp=mp, G=1.0)
+    a0 = np.stack(p.acc())
+    f.computeAccelerations(p)
+    assert np.array_equal(np.stack(p.acc()), a0)
+    for bpc in ("1", "4"):
+        monkeypatch.setenv("NB_WALK_BPC", bpc)
+        _, q = run_bh(nb, pl, eps=0.005, mp=mp, G=1.0)
+        assert np.array_equal(np.stack(q.acc()), a0), bpc
